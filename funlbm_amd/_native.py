@@ -1,0 +1,155 @@
+"""ctypes binding of libfunlbm_hip.so (the C ABI declared in include/funlbm.h).
+
+The library is required: importing a compute entry point without it raises
+RuntimeError, there is no CPU fallback.
+"""
+
+import ctypes
+import os
+import threading
+
+import torch
+
+from .build import LIB_PATH
+
+c_int32 = ctypes.c_int32
+c_float = ctypes.c_float
+c_void_p = ctypes.c_void_p
+c_int64 = ctypes.c_int64
+
+BC_PERIODICAL = 11000
+BC_WALL = 1200
+BC_WALL_WITH_SPEED = 1201
+
+STORE_F32 = 0
+STORE_F16 = 1
+STORE_BF16 = 2
+STORE_F64 = 3
+
+STORAGE_CODES = {"f32": STORE_F32, "f16": STORE_F16, "bf16": STORE_BF16, "f64": STORE_F64}
+STORAGE_DTYPES = {
+    STORE_F32: torch.float32,
+    STORE_F16: torch.float16,
+    STORE_BF16: torch.bfloat16,
+    STORE_F64: torch.float64,
+}
+
+# particle state layout (double[P][PSTRIDE])
+P_CX, P_CU, P_CW, P_CF, P_CT, P_ANGLE, P_ROT = 0, 3, 6, 9, 12, 15, 18
+P_MASS, P_INERTIA, P_RHO, P_GRAV = 27, 28, 31, 32
+PSTRIDE = 40
+
+EINVAL = -1
+EUNSUPPORTED_BC = -2
+EUNSUPPORTED_LATTICE = -3
+ETOOBIG = -4
+EFAILED = -5
+
+
+class Lattice(ctypes.Structure):
+    """funlbm_lattice of include/funlbm.h."""
+
+    _fields_ = [
+        ("q", c_int32),
+        ("nx", c_int32),
+        ("ny", c_int32),
+        ("nz", c_int32),
+        ("storage", c_int32),
+        ("bc", c_int32 * 6),
+        ("uzero_mask", c_int32),
+        ("tau", c_float),
+        ("rho0", c_float),
+        ("dev_scale", c_float),
+        ("body_force", c_float * 3),
+        ("uw", (c_float * 3) * 6),
+    ]
+
+
+P = c_void_p
+LATP = ctypes.POINTER(Lattice)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "funlbm_abi_version": (c_int32, []),
+    "funlbm_last_error": (ctypes.c_char_p, []),
+    "funlbm_state_bytes": (c_int32, [LATP, ctypes.POINTER(c_int64)]),
+    "funlbm_collide": (c_int32, [LATP, P, P, P, P, P]),
+    "funlbm_stream": (c_int32, [LATP, P, P, P, P]),
+    "funlbm_moments": (c_int32, [LATP, P, P, P, P, P, P]),
+    "funlbm_forcing": (c_int32, [LATP, P, P, P, P]),
+    "funlbm_pack": (c_int32, [LATP, P, P, P]),
+    "funlbm_equilibrium_state": (c_int32, [LATP, P, P, P, P]),
+    "funlbm_fused_step": (c_int32, [LATP, P, P, P, P, P, P, P]),
+    "funlbm_materialize": (c_int32, [LATP, P, P, P, P, P, P, P, P, P]),
+    "funlbm_copy_calibrate": (c_int32, [P, P, c_int64, P]),
+    "funlbm_ibm_interp": (
+        c_int32,
+        [LATP, P, P, P, P, P, c_int32, P, P, P, c_float, c_int32, P, P, P, P],
+    ),
+    "funlbm_wall_spring": (c_int32, [LATP, c_int32, P, P, P, c_float, P]),
+    "funlbm_particle_forces": (c_int32, [c_int32, P, P, P, P, P, ctypes.c_double, ctypes.c_double, P]),
+    "funlbm_particle_pose": (c_int32, [c_int32, P, P, P, P, c_int32, P]),
+    "funlbm_ibm_spread": (c_int32, [LATP, c_int32, P, P, P, P, P, P, P, P, P]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    """Loads the HIP library once; raises RuntimeError if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise RuntimeError(
+                    f"{LIB_PATH} is missing: build the HIP extension first "
+                    "(python -c 'import __graft_entry__ as g; g.build()')"
+                )
+            h = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in SIGNATURES.items():
+                fn = getattr(h, name)
+                fn.restype = res
+                fn.argtypes = args
+            if h.funlbm_abi_version() != 1:
+                raise RuntimeError("libfunlbm_hip.so has an unexpected ABI version")
+            _lib = h
+    return _lib
+
+
+def last_error() -> str:
+    msg = lib().funlbm_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int, what: str) -> None:
+    """Maps a C status to the reference's exception types."""
+    if rc == 0:
+        return
+    msg = f"{what}: {last_error()}"
+    if rc == EUNSUPPORTED_BC:
+        raise NotImplementedError(msg)
+    if rc in (EUNSUPPORTED_LATTICE, EINVAL, ETOOBIG):
+        raise ValueError(msg)
+    raise RuntimeError(f"{msg} (status {rc})")
+
+
+def ptr(t):
+    """Raw device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return c_void_p(t.data_ptr())
+
+
+def stream_handle(device=None):
+    return c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args), name)

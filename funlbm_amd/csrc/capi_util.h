@@ -1,0 +1,111 @@
+// Host-side helpers shared by the C-ABI entry points: thread-local error
+// text, argument validation and launch checks.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+
+#include <algorithm>
+
+#include "../../include/funlbm.h"
+#include "lbm_common.h"
+
+namespace funlbm {
+
+inline char* error_buffer() {
+  static thread_local char buf[512] = {0};
+  return buf;
+}
+
+inline const char* last_error_message() { return error_buffer(); }
+
+inline int set_error(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(error_buffer(), 512, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+inline int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error((int)e, "%s: %s", what, hipGetErrorString(e));
+  return 0;
+}
+
+inline int storage_elem_bytes(int storage) {
+  switch (storage) {
+    case FUNLBM_STORE_F32: return 4;
+    case FUNLBM_STORE_F16: return 2;
+    case FUNLBM_STORE_BF16: return 2;
+    case FUNLBM_STORE_F64: return 8;
+    default: return 0;
+  }
+}
+
+// Validates a lattice descriptor and fills the kernel parameters.
+inline int make_params(const funlbm_lattice* lat, KParams* p) {
+  if (!lat) return set_error(FUNLBM_EINVAL, "lattice descriptor is NULL");
+  if (lat->q != 13 && lat->q != 15 && lat->q != 19 && lat->q != 27)
+    return set_error(FUNLBM_EUNSUPPORTED_LATTICE, "unknown lattice D3Q%d", lat->q);
+  if (lat->nx < 1 || lat->ny < 1 || lat->nz < 1)
+    return set_error(FUNLBM_EINVAL, "bad grid %d x %d x %d", lat->nx, lat->ny, lat->nz);
+  if (storage_elem_bytes(lat->storage) == 0)
+    return set_error(FUNLBM_EINVAL, "unknown storage code %d", lat->storage);
+  if (!(lat->tau > 0.0f) || !isfinite(lat->tau))
+    return set_error(FUNLBM_EINVAL, "tau must be positive and finite, got %g", (double)lat->tau);
+  const bool dev = lat->storage == FUNLBM_STORE_F16 || lat->storage == FUNLBM_STORE_BF16;
+  if (dev && !(lat->dev_scale > 0.0f))
+    return set_error(FUNLBM_EINVAL, "deviation storage needs dev_scale > 0");
+  p->nx = lat->nx;
+  p->ny = lat->ny;
+  p->nz = lat->nz;
+  p->n = (int64_t)lat->nx * lat->ny * lat->nz;
+  p->has_moving = 0;
+  for (int f = 0; f < 6; ++f) {
+    const int bc = lat->bc[f];
+    if (bc != FUNLBM_BC_PERIODICAL && bc != FUNLBM_BC_WALL && bc != FUNLBM_BC_WALL_WITH_SPEED)
+      return set_error(FUNLBM_EUNSUPPORTED_BC, "boundary code %d on face %d is not implemented",
+                       bc, f);
+    p->bc[f] = bc;
+    p->has_moving |= (bc == FUNLBM_BC_WALL_WITH_SPEED);
+    for (int d = 0; d < 3; ++d) p->uw[f][d] = lat->uw[f][d];
+  }
+  p->uzero = lat->uzero_mask & 0x3f;
+  p->tau = lat->tau;
+  p->rho0 = lat->rho0;
+  p->scale = dev ? lat->dev_scale : 1.0f;
+  p->inv_scale = dev ? 1.0f / lat->dev_scale : 1.0f;
+  p->has_body = 0;
+  for (int d = 0; d < 3; ++d) {
+    p->g[d] = lat->body_force[d];
+    p->has_body |= (lat->body_force[d] != 0.0f);
+  }
+  return 0;
+}
+
+template <class Fn>
+inline int dispatch_lattice(int q, Fn&& fn) {
+  switch (q) {
+    case 13: return fn(D3Q13{});
+    case 15: return fn(D3Q15{});
+    case 19: return fn(D3Q19{});
+    case 27: return fn(D3Q27{});
+    default: return set_error(FUNLBM_EUNSUPPORTED_LATTICE, "unknown lattice D3Q%d", q);
+  }
+}
+
+template <class Fn>
+inline int dispatch_storage(int storage, Fn&& fn) {
+  switch (storage) {
+    case FUNLBM_STORE_F32: return fn(StoreF32{});
+    case FUNLBM_STORE_F16: return fn(StoreF16{});
+    case FUNLBM_STORE_BF16: return fn(StoreBF16{});
+    case FUNLBM_STORE_F64: return fn(StoreF64{});
+    default: return set_error(FUNLBM_EINVAL, "unknown storage code %d", storage);
+  }
+}
+
+}  // namespace funlbm

@@ -1,0 +1,386 @@
+// Fluid kernels of the D3Qn BGK timestep and their C-ABI entry points.
+//
+// Layout: populations f[k][x][y][z] (z fastest), one thread per node over a
+// flattened 1-D grid of 256-thread blocks, so rows of any length keep every
+// lane busy and every access to a direction plane is a coalesced row.
+#include "capi_util.h"
+#include "lbm_common.h"
+
+namespace funlbm {
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------
+// Explicit (reference-structured) kernels, float, one pass each.
+// ---------------------------------------------------------------------------
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_collide(KParams p, float* __restrict__ f,
+                                                    const float* __restrict__ u,
+                                                    const float* __restrict__ rho,
+                                                    float* __restrict__ feq) {
+  const int64_t node = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (node >= p.n) return;
+  float fl[L::Q], e[L::Q];
+#pragma unroll
+  for (int k = 0; k < L::Q; ++k) fl[k] = f[(int64_t)k * p.n + node];
+  const float uu[3] = {u[node], u[p.n + node], u[2 * p.n + node]};
+  collide<L, float>(fl, uu, rho[node], p.tau, e);
+#pragma unroll
+  for (int k = 0; k < L::Q; ++k) {
+    f[(int64_t)k * p.n + node] = fl[k];
+    if (feq) feq[(int64_t)k * p.n + node] = e[k];
+  }
+}
+
+template <class L, class S>
+__global__ __launch_bounds__(kBlock) void k_stream(KParams p, const typename S::elem* __restrict__ src,
+                                                   typename S::elem* __restrict__ dst,
+                                                   const float* __restrict__ rho) {
+  using T = typename S::T;
+  const int64_t node = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (node >= p.n) return;
+  const Node c = node_coords(p, node);
+  static_for<0, L::Q>([&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    const Src s = source_of<L, K>(p, c);
+    typename S::elem v = src[(int64_t)s.k * p.n + lin(p, s.x, s.y, s.z)];
+    if (s.moving_face >= 0) {
+      const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
+      T val = S::load(v, wr, p.inv_scale, p.scale);
+      val = val - moving_wall_term<L, K, T>(p, s.moving_face, (T)rho[node]);
+      v = S::store(val, wr, p.inv_scale, p.scale);
+    }
+    dst[(int64_t)K * p.n + node] = v;
+  });
+}
+
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_moments(KParams p, const float* __restrict__ f,
+                                                    const float* __restrict__ fol,
+                                                    float* __restrict__ u, float* __restrict__ rho,
+                                                    float* __restrict__ pr) {
+  const int64_t node = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (node >= p.n) return;
+  const Node c = node_coords(p, node);
+  float fl[L::Q];
+#pragma unroll
+  for (int k = 0; k < L::Q; ++k) fl[k] = f[(int64_t)k * p.n + node];
+  float F[3] = {p.g[0], p.g[1], p.g[2]};
+  if (fol) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) F[d] = fol[(int64_t)d * p.n + node] + p.g[d];
+  }
+  float r, uu[3];
+  moments<L, float>(p, c, fl, F, r, uu);
+  rho[node] = r;
+  if (pr) pr[node] = r / 3.0f;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) u[(int64_t)d * p.n + node] = uu[d];
+}
+
+template <class L>
+__global__ __launch_bounds__(kBlock) void k_forcing(KParams p, float* __restrict__ f,
+                                                    const float* __restrict__ u,
+                                                    const float* __restrict__ fol) {
+  const int64_t node = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (node >= p.n) return;
+  float F[3] = {p.g[0], p.g[1], p.g[2]};
+  if (fol) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) F[d] = fol[(int64_t)d * p.n + node] + p.g[d];
+  }
+  float fl[L::Q];
+#pragma unroll
+  for (int k = 0; k < L::Q; ++k) fl[k] = f[(int64_t)k * p.n + node];
+  const float uu[3] = {u[node], u[p.n + node], u[2 * p.n + node]};
+  add_forcing<L, float>(fl, uu, F, p.tau);
+#pragma unroll
+  for (int k = 0; k < L::Q; ++k) f[(int64_t)k * p.n + node] = fl[k];
+}
+
+template <class L, class S>
+__global__ __launch_bounds__(kBlock) void k_pack(KParams p, const float* __restrict__ f,
+                                                 typename S::elem* __restrict__ state) {
+  const int64_t node = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (node >= p.n) return;
+  static_for<0, L::Q>([&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
+    state[(int64_t)K * p.n + node] =
+        S::store((typename S::T)f[(int64_t)K * p.n + node], wr, p.inv_scale, p.scale);
+  });
+}
+
+// State = equilibrium populations of (rho, u) computed in the storage's
+// arithmetic type (NULL rho = 1, NULL u = 0).
+template <class L, class S>
+__global__ __launch_bounds__(kBlock) void k_equilibrium(KParams p, const float* __restrict__ rho,
+                                                        const float* __restrict__ u,
+                                                        typename S::elem* __restrict__ state) {
+  using T = typename S::T;
+  const int64_t node = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (node >= p.n) return;
+  const T r = rho ? (T)rho[node] : (T)1;
+  T uu[3] = {(T)0, (T)0, (T)0};
+  if (u) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) uu[d] = (T)u[(int64_t)d * p.n + node];
+  }
+  T f[L::Q], e[L::Q];
+#pragma unroll
+  for (int k = 0; k < L::Q; ++k) f[k] = (T)0;
+  collide<L, T>(f, uu, r, (T)1, e);
+  static_for<0, L::Q>([&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
+    state[(int64_t)K * p.n + node] = S::store(e[K], wr, p.inv_scale, p.scale);
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Fused pull step.  FORCE: the lattice has a forcing field (IBM FOL and/or a
+// body force).  MAT: write the reference's f/u/rho/p instead of colliding.
+// ---------------------------------------------------------------------------
+template <class L, class S, bool FORCE, bool MAT>
+__global__ __launch_bounds__(kBlock) void k_fused(
+    KParams p, const typename S::elem* __restrict__ src, typename S::elem* __restrict__ dst,
+    const float* __restrict__ fol_old, const float* __restrict__ fol_new,
+    const float* __restrict__ rho_prev, float* __restrict__ rho_next, float* __restrict__ out_f,
+    float* __restrict__ out_u, float* __restrict__ out_rho, float* __restrict__ out_p) {
+  using T = typename S::T;
+  const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t node = (int64_t)b * kBlock + threadIdx.x;
+  if (node >= p.n) return;
+  const Node c = node_coords(p, node);
+  T f[L::Q];
+  pull_node<L, S>(p, src, c, node, rho_prev, f);
+
+  T Fn[3] = {(T)0, (T)0, (T)0};
+  if (FORCE) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      Fn[d] = (T)(fol_new ? fol_new[(int64_t)d * p.n + node] + p.g[d] : p.g[d]);
+    if (Fn[0] != (T)0 || Fn[1] != (T)0 || Fn[2] != (T)0) {
+      // u' of the first moment update (old FOL), then cul_equ2 with the new FOL.
+      T Fo[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        Fo[d] = (T)(fol_old ? fol_old[(int64_t)d * p.n + node] + p.g[d] : p.g[d]);
+      T r1, u1[3];
+      moments<L, T>(p, c, f, Fo, r1, u1);
+      add_forcing<L, T>(f, u1, Fn, (T)p.tau);
+    }
+  }
+  T rho, u[3];
+  moments<L, T>(p, c, f, Fn, rho, u);
+
+  if constexpr (MAT) {
+    if (out_f) {
+#pragma unroll
+      for (int k = 0; k < L::Q; ++k) out_f[(int64_t)k * p.n + node] = (float)f[k];
+    }
+    if (out_u) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) out_u[(int64_t)d * p.n + node] = (float)u[d];
+    }
+    if (out_rho) out_rho[node] = (float)rho;
+    if (out_p) out_p[node] = (float)(rho / (T)3);
+    return;
+  } else {
+    collide<L, T>(f, u, rho, (T)p.tau, (T*)nullptr);
+    if (p.has_moving && rho_next) {
+      bool mf = false;
+#pragma unroll
+      for (int face = 0; face < 6; ++face)
+        mf |= (p.bc[face] == FUNLBM_BC_WALL_WITH_SPEED) && on_face(p, c, face);
+      if (mf) rho_next[node] = (float)rho;
+    }
+    static_for<0, L::Q>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
+      dst[(int64_t)K * p.n + node] = S::store(f[K], wr, p.inv_scale, p.scale);
+    });
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy(const uint4* __restrict__ src,
+                                                 uint4* __restrict__ dst, int64_t n16) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+// ---------------------------------------------------------------------------
+static int grid_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
+
+}  // namespace funlbm
+
+using namespace funlbm;
+
+extern "C" {
+
+int funlbm_abi_version(void) { return FUNLBM_ABI_VERSION; }
+
+const char* funlbm_last_error(void) { return last_error_message(); }
+
+int funlbm_state_bytes(const funlbm_lattice* lat, int64_t* bytes) {
+  KParams p;
+  int rc = make_params(lat, &p);
+  if (rc) return rc;
+  if (!bytes) return set_error(FUNLBM_EINVAL, "bytes is NULL");
+  *bytes = (int64_t)lat->q * p.n * (int64_t)storage_elem_bytes(lat->storage);
+  return 0;
+}
+
+int funlbm_collide(const funlbm_lattice* lat, float* f, const float* u, const float* rho,
+                   float* feq, void* stream) {
+  KParams p;
+  int rc = make_params(lat, &p);
+  if (rc) return rc;
+  if (!f || !u || !rho) return set_error(FUNLBM_EINVAL, "collide: NULL array");
+  hipStream_t s = (hipStream_t)stream;
+  return dispatch_lattice(lat->q, [&](auto l) {
+    using L = decltype(l);
+    hipLaunchKernelGGL(k_collide<L>, dim3(grid_for(p.n)), dim3(kBlock), 0, s, p, f, u, rho, feq);
+    return check_launch("collide");
+  });
+}
+
+int funlbm_stream(const funlbm_lattice* lat, const void* f_src, void* f_dst, const float* rho,
+                  void* stream) {
+  KParams p;
+  int rc = make_params(lat, &p);
+  if (rc) return rc;
+  if (!f_src || !f_dst) return set_error(FUNLBM_EINVAL, "stream: NULL array");
+  if (f_src == f_dst) return set_error(FUNLBM_EINVAL, "stream: source and destination alias");
+  if (p.has_moving && !rho) return set_error(FUNLBM_EINVAL, "stream: moving wall needs rho");
+  hipStream_t s = (hipStream_t)stream;
+  return dispatch_lattice(lat->q, [&](auto l) {
+    using L = decltype(l);
+    return dispatch_storage(lat->storage, [&](auto st) {
+      using S = decltype(st);
+      using E = typename S::elem;
+      hipLaunchKernelGGL((k_stream<L, S>), dim3(grid_for(p.n)), dim3(kBlock), 0, s, p,
+                         (const E*)f_src, (E*)f_dst, rho);
+      return check_launch("stream");
+    });
+  });
+}
+
+int funlbm_moments(const funlbm_lattice* lat, const float* f, const float* fol, float* u,
+                   float* rho, float* pr, void* stream) {
+  KParams p;
+  int rc = make_params(lat, &p);
+  if (rc) return rc;
+  if (!f || !u || !rho) return set_error(FUNLBM_EINVAL, "moments: NULL array");
+  hipStream_t s = (hipStream_t)stream;
+  return dispatch_lattice(lat->q, [&](auto l) {
+    using L = decltype(l);
+    hipLaunchKernelGGL(k_moments<L>, dim3(grid_for(p.n)), dim3(kBlock), 0, s, p, f, fol, u, rho, pr);
+    return check_launch("moments");
+  });
+}
+
+int funlbm_forcing(const funlbm_lattice* lat, float* f, const float* u, const float* fol,
+                   void* stream) {
+  KParams p;
+  int rc = make_params(lat, &p);
+  if (rc) return rc;
+  if (!f || !u) return set_error(FUNLBM_EINVAL, "forcing: NULL array");
+  hipStream_t s = (hipStream_t)stream;
+  return dispatch_lattice(lat->q, [&](auto l) {
+    using L = decltype(l);
+    hipLaunchKernelGGL(k_forcing<L>, dim3(grid_for(p.n)), dim3(kBlock), 0, s, p, f, u, fol);
+    return check_launch("forcing");
+  });
+}
+
+int funlbm_pack(const funlbm_lattice* lat, const float* f, void* state, void* stream) {
+  KParams p;
+  int rc = make_params(lat, &p);
+  if (rc) return rc;
+  if (!f || !state) return set_error(FUNLBM_EINVAL, "pack: NULL array");
+  hipStream_t s = (hipStream_t)stream;
+  return dispatch_lattice(lat->q, [&](auto l) {
+    using L = decltype(l);
+    return dispatch_storage(lat->storage, [&](auto st) {
+      using S = decltype(st);
+      hipLaunchKernelGGL((k_pack<L, S>), dim3(grid_for(p.n)), dim3(kBlock), 0, s, p, f,
+                         (typename S::elem*)state);
+      return check_launch("pack");
+    });
+  });
+}
+
+int funlbm_equilibrium_state(const funlbm_lattice* lat, const float* rho, const float* u,
+                             void* state, void* stream) {
+  KParams p;
+  int rc = make_params(lat, &p);
+  if (rc) return rc;
+  if (!state) return set_error(FUNLBM_EINVAL, "equilibrium: NULL state");
+  hipStream_t s = (hipStream_t)stream;
+  return dispatch_lattice(lat->q, [&](auto l) {
+    using L = decltype(l);
+    return dispatch_storage(lat->storage, [&](auto st) {
+      using S = decltype(st);
+      hipLaunchKernelGGL((k_equilibrium<L, S>), dim3(grid_for(p.n)), dim3(kBlock), 0, s, p, rho, u,
+                         (typename S::elem*)state);
+      return check_launch("equilibrium");
+    });
+  });
+}
+
+static int fused_launch(const funlbm_lattice* lat, const void* src, void* dst, const float* fol_old,
+                        const float* fol_new, const float* rho_prev, float* rho_next, float* out_f,
+                        float* out_u, float* out_rho, float* out_p, bool mat, void* stream) {
+  KParams p;
+  int rc = make_params(lat, &p);
+  if (rc) return rc;
+  if (!src || (!mat && !dst)) return set_error(FUNLBM_EINVAL, "fused: NULL state");
+  if (!mat && src == dst) return set_error(FUNLBM_EINVAL, "fused: source and destination alias");
+  if (p.has_moving && !rho_prev)
+    return set_error(FUNLBM_EINVAL, "fused: moving wall needs the previous face density");
+  const bool force = fol_old || fol_new || p.has_body;
+  hipStream_t s = (hipStream_t)stream;
+  return dispatch_lattice(lat->q, [&](auto l) {
+    using L = decltype(l);
+    return dispatch_storage(lat->storage, [&](auto st) {
+      using S = decltype(st);
+      using E = typename S::elem;
+      auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(grid_for(p.n)), dim3(kBlock), 0, s, p, (const E*)src, (E*)dst,
+                           fol_old, fol_new, rho_prev, rho_next, out_f, out_u, out_rho, out_p);
+        return check_launch(mat ? "materialize" : "fused_step");
+      };
+      if (mat) return force ? go(k_fused<L, S, true, true>) : go(k_fused<L, S, false, true>);
+      return force ? go(k_fused<L, S, true, false>) : go(k_fused<L, S, false, false>);
+    });
+  });
+}
+
+int funlbm_fused_step(const funlbm_lattice* lat, const void* state_src, void* state_dst,
+                      const float* fol_old, const float* fol_new, const float* rho_prev,
+                      float* rho_next, void* stream) {
+  return fused_launch(lat, state_src, state_dst, fol_old, fol_new, rho_prev, rho_next, nullptr,
+                      nullptr, nullptr, nullptr, false, stream);
+}
+
+int funlbm_materialize(const funlbm_lattice* lat, const void* state_src, const float* fol_old,
+                       const float* fol_new, const float* rho_prev, float* f, float* u, float* rho,
+                       float* pr, void* stream) {
+  return fused_launch(lat, state_src, nullptr, fol_old, fol_new, rho_prev, nullptr, f, u, rho, pr,
+                      true, stream);
+}
+
+int funlbm_copy_calibrate(const void* src, void* dst, int64_t n_bytes, void* stream) {
+  if (!src || !dst || n_bytes <= 0 || (n_bytes % 16) != 0)
+    return set_error(FUNLBM_EINVAL, "copy: bad arguments");
+  const int64_t n16 = n_bytes / 16;
+  int blocks = (int)std::min<int64_t>((n16 + kBlock - 1) / kBlock, 256 * 64);
+  hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(kBlock), 0, (hipStream_t)stream,
+                     (const uint4*)src, (uint4*)dst, n16);
+  return check_launch("copy");
+}
+
+}  // extern "C"

@@ -1,0 +1,390 @@
+// Lattice tables, storage formats and the per-node arithmetic shared by every
+// fluid and IBM kernel of the library.  gfx950 only.
+//
+// The per-node arithmetic follows the operation order of the reference so that
+// float results agree to rounding (src/funlbm/flow/d3/base.py:103-159):
+// no fused multiply-add is formed (the library is built with -ffp-contract=off),
+// divisions stay divisions and sums run in direction order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "../../include/funlbm.h"
+
+namespace funlbm {
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [B, E).
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Lattice tables.  Direction order, velocities and weights are the reference's
+// (src/funlbm/flow/d3/d3q13.py:7-44, d3q15.py:7-48, d3q19.py:7-60,
+// d3q27.py:7-77); REV[k] is the index of -e_k (the reference builds it from a
+// string map, d3q19.py:55-60).
+// ---------------------------------------------------------------------------
+struct D3Q13 {
+  static constexpr int Q = 13;
+  static constexpr int E[13][3] = {{0, 0, 0},  {1, 1, 0},  {1, -1, 0},  {1, 0, 1},  {1, 0, -1},
+                                   {0, 1, 1},  {0, 1, -1}, {-1, -1, 0}, {-1, 1, 0}, {-1, 0, -1},
+                                   {-1, 0, 1}, {0, -1, -1}, {0, -1, 1}};
+  static constexpr double W[13] = {1.0 / 2,  1.0 / 24, 1.0 / 24, 1.0 / 24, 1.0 / 24,
+                                   1.0 / 24, 1.0 / 24, 1.0 / 24, 1.0 / 24, 1.0 / 24,
+                                   1.0 / 24, 1.0 / 24, 1.0 / 24};
+  static constexpr int REV[13] = {0, 7, 8, 9, 10, 11, 12, 1, 2, 3, 4, 5, 6};
+};
+
+struct D3Q15 {
+  static constexpr int Q = 15;
+  static constexpr int E[15][3] = {{0, 0, 0},   {1, 0, 0},   {-1, 0, 0},  {0, 1, 0},  {0, -1, 0},
+                                   {0, 0, 1},   {0, 0, -1},  {1, 1, 1},   {-1, 1, 1}, {1, -1, 1},
+                                   {-1, -1, 1}, {1, 1, -1},  {-1, 1, -1}, {1, -1, -1}, {-1, -1, -1}};
+  static constexpr double W[15] = {2.0 / 9,  1.0 / 9,  1.0 / 9,  1.0 / 9,  1.0 / 9,
+                                   1.0 / 9,  1.0 / 9,  1.0 / 72, 1.0 / 72, 1.0 / 72,
+                                   1.0 / 72, 1.0 / 72, 1.0 / 72, 1.0 / 72, 1.0 / 72};
+  static constexpr int REV[15] = {0, 2, 1, 4, 3, 6, 5, 14, 13, 12, 11, 10, 9, 8, 7};
+};
+
+struct D3Q19 {
+  static constexpr int Q = 19;
+  static constexpr int E[19][3] = {{0, 0, 0},  {1, 0, 0},   {-1, 0, 0}, {0, 1, 0},  {0, -1, 0},
+                                   {0, 0, 1},  {0, 0, -1},  {1, 1, 0},  {-1, 1, 0}, {1, -1, 0},
+                                   {-1, -1, 0}, {0, 1, 1},  {0, -1, 1}, {0, 1, -1}, {0, -1, -1},
+                                   {1, 0, 1},  {-1, 0, 1},  {1, 0, -1}, {-1, 0, -1}};
+  static constexpr double W[19] = {1.0 / 3,  1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18,
+                                   1.0 / 18, 1.0 / 18, 1.0 / 36, 1.0 / 36, 1.0 / 36,
+                                   1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36,
+                                   1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
+  static constexpr int REV[19] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
+};
+
+struct D3Q27 {
+  static constexpr int Q = 27;
+  static constexpr int E[27][3] = {
+      {0, 0, 0},   {1, 0, 0},   {-1, 0, 0}, {0, 1, 0},   {0, -1, 0},  {0, 0, 1},  {0, 0, -1},
+      {1, 1, 0},   {-1, 1, 0},  {1, -1, 0}, {-1, -1, 0}, {0, 1, 1},   {0, -1, 1}, {0, 1, -1},
+      {0, -1, -1}, {1, 0, 1},   {-1, 0, 1}, {1, 0, -1},  {-1, 0, -1}, {1, 1, 1},  {1, 1, -1},
+      {1, -1, 1},  {1, -1, -1}, {-1, 1, 1}, {-1, 1, -1}, {-1, -1, 1}, {-1, -1, -1}};
+  static constexpr double W[27] = {
+      8.0 / 27,  2.0 / 27,  2.0 / 27,  2.0 / 27,  2.0 / 27,  2.0 / 27,  2.0 / 27,
+      1.0 / 54,  1.0 / 54,  1.0 / 54,  1.0 / 54,  1.0 / 54,  1.0 / 54,  1.0 / 54,
+      1.0 / 54,  1.0 / 54,  1.0 / 54,  1.0 / 54,  1.0 / 54,  1.0 / 216, 1.0 / 216,
+      1.0 / 216, 1.0 / 216, 1.0 / 216, 1.0 / 216, 1.0 / 216, 1.0 / 216};
+  static constexpr int REV[27] = {0,  2,  1,  4,  3,  6,  5,  10, 9,  8,  7,  14, 13, 12,
+                                  11, 18, 17, 16, 15, 26, 25, 24, 23, 22, 21, 20, 19};
+};
+
+// ---------------------------------------------------------------------------
+// Storage formats of the population state.
+//   T      arithmetic type
+//   elem   stored element
+//   load   stored -> T for direction weight w
+//   store  T -> stored
+// Deviation formats keep (f - w*rho0)*scale; loads and stores are a
+// multiply-add away from f, so streaming a deviation buffer is still a pure
+// permutation of stored bits.
+// ---------------------------------------------------------------------------
+struct StoreF32 {
+  using T = float;
+  using elem = float;
+  static constexpr int code = FUNLBM_STORE_F32;
+  __device__ static inline float load(elem v, float, float, float) { return v; }
+  __device__ static inline elem store(float f, float, float, float) { return f; }
+};
+
+struct StoreF64 {
+  using T = double;
+  using elem = double;
+  static constexpr int code = FUNLBM_STORE_F64;
+  __device__ static inline double load(elem v, double, double, double) { return v; }
+  __device__ static inline elem store(double f, double, double, double) { return f; }
+};
+
+struct StoreF16 {
+  using T = float;
+  using elem = __half;
+  static constexpr int code = FUNLBM_STORE_F16;
+  __device__ static inline float load(elem v, float wrho0, float inv_scale, float) {
+    return __half2float(v) * inv_scale + wrho0;
+  }
+  __device__ static inline elem store(float f, float wrho0, float, float scale) {
+    return __float2half_rn((f - wrho0) * scale);
+  }
+};
+
+struct StoreBF16 {
+  using T = float;
+  using elem = __hip_bfloat16;
+  static constexpr int code = FUNLBM_STORE_BF16;
+  __device__ static inline float load(elem v, float wrho0, float inv_scale, float) {
+    return __bfloat162float(v) * inv_scale + wrho0;
+  }
+  __device__ static inline elem store(float f, float wrho0, float, float scale) {
+    return __float2bfloat16(((f - wrho0) * scale));
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Kernel-side lattice parameters, built from funlbm_lattice on the host.
+// ---------------------------------------------------------------------------
+struct KParams {
+  int nx, ny, nz;
+  int64_t n;  // nodes
+  int bc[6];
+  int uzero;
+  int has_moving;  // any WALL_WITH_SPEED face
+  int has_body;    // body_force != 0
+  float tau;
+  float rho0, scale, inv_scale;
+  float g[3];
+  float uw[6][3];
+};
+
+// Constants of the reference: cs = float32(sqrt(1/3)), cs^2 and cs^4 rounded
+// to float32 as torch computes `cs**2` and `cs2**2` on float32 scalars
+// (src/funlbm/flow/base.py:37-39, flow/d3/base.py:106-107,152-156).
+template <class T>
+struct Cs;
+template <>
+struct Cs<float> {
+  __device__ static inline float cs2() { return __uint_as_float(0x3eaaaaaau); }  // fl32(cs*cs)
+  __device__ static inline float cs4() { return cs2() * cs2(); }
+};
+template <>
+struct Cs<double> {
+  __device__ static inline double cs2() { return 1.0 / 3.0; }
+  __device__ static inline double cs4() { return 1.0 / 9.0; }
+};
+
+template <class T>
+__device__ inline T wt(double w) {
+  return (T)w;
+}
+
+// Node coordinates from a flattened node index (z fastest).
+struct Node {
+  int x, y, z;
+};
+__device__ inline Node node_coords(const KParams& p, int64_t node) {
+  Node c;
+  int64_t t = node / p.nz;
+  c.z = (int)(node - t * p.nz);
+  c.y = (int)(t % p.ny);
+  c.x = (int)(t / p.ny);
+  return c;
+}
+
+__device__ inline bool on_face(const KParams& p, const Node& c, int face) {
+  switch (face) {
+    case 0: return c.x == 0;
+    case 1: return c.x == p.nx - 1;
+    case 2: return c.y == 0;
+    case 3: return c.y == p.ny - 1;
+    case 4: return c.z == 0;
+    default: return c.z == p.nz - 1;
+  }
+}
+
+__device__ inline bool uzero_at(const KParams& p, const Node& c) {
+  if (!p.uzero) return false;
+  bool z = false;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) z |= ((p.uzero >> f) & 1) && on_face(p, c, f);
+  return z;
+}
+
+// ---------------------------------------------------------------------------
+// Source of population k at node c after streaming.  The reference streams by
+// torch.roll (push: f[x] = f*[x - e_k], periodic wrap) and then overrides the
+// six faces in the order input, output, back, forward, bottom, top; a later
+// override wins (src/funlbm/flow/d3/base.py:161-323).  Per face and its
+// incoming directions (e_axis = +1 on the low face, -1 on the high face):
+//   PERIODICAL       f*[opposite face, same transverse coordinates, k]
+//   WALL             f*[same node, rev(k)]
+//   WALL_WITH_SPEED  f*[same node, rev(k)] - 2 rho w_k (e_k . uw)/cs^2
+// ---------------------------------------------------------------------------
+struct Src {
+  int x, y, z, k;
+  int moving_face;  // -1 or the face whose wall velocity corrects the value
+};
+
+template <class L, int K>
+__device__ inline Src source_of(const KParams& p, const Node& c) {
+  constexpr int ex = L::E[K][0], ey = L::E[K][1], ez = L::E[K][2];
+  Src s;
+  s.k = K;
+  s.moving_face = -1;
+  s.x = c.x - ex;
+  s.y = c.y - ey;
+  s.z = c.z - ez;
+  if (ex == 1 && c.x == 0) s.x = p.nx - 1;
+  if (ex == -1 && c.x == p.nx - 1) s.x = 0;
+  if (ey == 1 && c.y == 0) s.y = p.ny - 1;
+  if (ey == -1 && c.y == p.ny - 1) s.y = 0;
+  if (ez == 1 && c.z == 0) s.z = p.nz - 1;
+  if (ez == -1 && c.z == p.nz - 1) s.z = 0;
+
+  auto apply = [&](int face, int ox, int oy, int oz) {
+    const int bc = p.bc[face];
+    if (bc == FUNLBM_BC_PERIODICAL) {
+      s.x = ox; s.y = oy; s.z = oz; s.k = K; s.moving_face = -1;
+    } else {  // WALL or WALL_WITH_SPEED (others are rejected on the host)
+      s.x = c.x; s.y = c.y; s.z = c.z; s.k = L::REV[K];
+      s.moving_face = (bc == FUNLBM_BC_WALL_WITH_SPEED) ? face : -1;
+    }
+  };
+  if (ex == 1 && c.x == 0) apply(0, p.nx - 1, c.y, c.z);
+  if (ex == -1 && c.x == p.nx - 1) apply(1, 0, c.y, c.z);
+  if (ey == 1 && c.y == 0) apply(2, c.x, p.ny - 1, c.z);
+  if (ey == -1 && c.y == p.ny - 1) apply(3, c.x, 0, c.z);
+  if (ez == 1 && c.z == 0) apply(4, c.x, c.y, p.nz - 1);
+  if (ez == -1 && c.z == p.nz - 1) apply(5, c.x, c.y, 0);
+  return s;
+}
+
+__device__ inline int64_t lin(const KParams& p, int x, int y, int z) {
+  return ((int64_t)x * p.ny + y) * p.nz + z;
+}
+
+// Moving-wall correction 2 (rho w_k) (e_k . uw) / cs^2 in the reference's
+// operation order (src/funlbm/flow/d3/base.py:204-213).
+template <class L, int K, class T>
+__device__ inline T moving_wall_term(const KParams& p, int face, T rho) {
+  constexpr int ex = L::E[K][0], ey = L::E[K][1], ez = L::E[K][2];
+  T eu = (T)0;
+  eu = eu + (T)ex * (T)p.uw[face][0];
+  eu = eu + (T)ey * (T)p.uw[face][1];
+  eu = eu + (T)ez * (T)p.uw[face][2];
+  T tmp = (T)2 * (rho * wt<T>(L::W[K]));
+  return tmp * eu / Cs<T>::cs2();
+}
+
+// Pull all Q post-stream populations of node c out of a state buffer.
+template <class L, class S>
+__device__ inline void pull_node(const KParams& p, const typename S::elem* __restrict__ src,
+                                 const Node& c, int64_t node, const float* __restrict__ rho_prev,
+                                 typename S::T* f) {
+  using T = typename S::T;
+  static_for<0, L::Q>([&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    const Src s = source_of<L, K>(p, c);
+    const int64_t idx = (int64_t)s.k * p.n + lin(p, s.x, s.y, s.z);
+    // The element was stored with the weight of s.k; rev(k) and k share a
+    // weight on every lattice.
+    T v = S::load(src[idx], (float)(wt<float>(L::W[K]) * p.rho0), p.inv_scale, p.scale);
+    if (s.moving_face >= 0) v = v - moving_wall_term<L, K, T>(p, s.moving_face, (T)rho_prev[node]);
+    f[K] = v;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Moments (src/funlbm/flow/d3/base.py:116-133): rho = sum_k f_k in direction
+// order; j = f.e in direction order; u = (j + F/2)/rho; u := 0 on masked faces.
+// ---------------------------------------------------------------------------
+template <class L, class T>
+__device__ inline void moments(const KParams& p, const Node& c, const T* f, const T F[3], T& rho,
+                               T u[3]) {
+  T r = (T)0, jx = (T)0, jy = (T)0, jz = (T)0;
+#pragma unroll
+  for (int k = 0; k < L::Q; ++k) {
+    r = r + f[k];
+    if (L::E[k][0] == 1) jx = jx + f[k];
+    if (L::E[k][0] == -1) jx = jx - f[k];
+    if (L::E[k][1] == 1) jy = jy + f[k];
+    if (L::E[k][1] == -1) jy = jy - f[k];
+    if (L::E[k][2] == 1) jz = jz + f[k];
+    if (L::E[k][2] == -1) jz = jz - f[k];
+  }
+  rho = r;
+  u[0] = (jx + F[0] / (T)2) / r;
+  u[1] = (jy + F[1] / (T)2) / r;
+  u[2] = (jz + F[2] / (T)2) / r;
+  if (uzero_at(p, c)) {
+    u[0] = (T)0;
+    u[1] = (T)0;
+    u[2] = (T)0;
+  }
+}
+
+// e_k . u in the reference's summation order (x, y, z), exact up to the two
+// additions; tmp_k = (e_k . u)/cs^2.
+template <class L, int K, class T>
+__device__ inline T edotu(const T u[3]) {
+  T d = (T)0;
+  if (L::E[K][0] == 1) d = d + u[0];
+  if (L::E[K][0] == -1) d = d - u[0];
+  if (L::E[K][1] == 1) d = d + u[1];
+  if (L::E[K][1] == -1) d = d - u[1];
+  if (L::E[K][2] == 1) d = d + u[2];
+  if (L::E[K][2] == -1) d = d - u[2];
+  return d;
+}
+
+template <class T>
+__device__ inline T sqrt_t(T v);
+template <>
+__device__ inline float sqrt_t<float>(float v) {
+  return __fsqrt_rn(v);
+}
+template <>
+__device__ inline double sqrt_t<double>(double v) {
+  return __dsqrt_rn(v);
+}
+
+// BGK collision (src/funlbm/flow/d3/base.py:150-159).  feq_k =
+// (1 + tmp + tmp^2/2 - u2) * (rho w_k), tmp = (u.e_k)/cs^2,
+// u2 = |u|^2/cs^2/2 with |u| = sqrt(u.u) squared again as torch does.
+// f_k <- f_k - (f_k - feq_k)/tau.  If feq != nullptr it receives feq_k.
+template <class L, class T>
+__device__ inline void collide(T* f, const T u[3], T rho, T tau, T* feq) {
+  const T nrm = sqrt_t<T>(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  const T u2 = nrm * nrm / Cs<T>::cs2() / (T)2;
+  static_for<0, L::Q>([&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    const T tmp = edotu<L, K, T>(u) / Cs<T>::cs2();
+    const T e = ((((T)1 + tmp) + tmp * tmp / (T)2) - u2) * (rho * wt<T>(L::W[K]));
+    if (feq) feq[K] = e;
+    f[K] = f[K] - (f[K] - e) / tau;
+  });
+}
+
+// Forcing term of FlowD3.cul_equ2 (src/funlbm/flow/d3/base.py:103-114):
+// t1 = (e_a - u)/cs^2, t2 = (u e_a) e_a / cs^4 (elementwise),
+// t3 = sum_d (t1 + t2)_d F_d, f_a += ((1 - 1/(2 tau)) w_a) t3.
+template <class L, class T>
+__device__ inline void add_forcing(T* f, const T u[3], const T F[3], T tau) {
+  const T omega = (T)1 - (T)1 / ((T)2 * tau);
+  static_for<0, L::Q>([&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    T t3 = (T)0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const T e = (T)L::E[K][d];
+      const T t1 = (e - u[d]) / Cs<T>::cs2();
+      const T t2 = ((u[d] * e) * e) / Cs<T>::cs4();
+      t3 = (d == 0) ? (t1 + t2) * F[d] : t3 + (t1 + t2) * F[d];
+    }
+    f[K] = f[K] + (omega * wt<T>(L::W[K])) * t3;
+  });
+}
+
+// Block index remap so that consecutive logical blocks share one XCD (and its
+// L2): the hardware hands out workgroups round-robin over the 8 XCDs.
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t nblocks) {
+  constexpr uint32_t NX = 8;
+  const uint32_t per = nblocks / NX;
+  if (b >= per * NX) return b;  // the ragged tail keeps its index
+  return (b % NX) * per + b / NX;
+}
+
+}  // namespace funlbm

@@ -1,0 +1,159 @@
+"""Solver configuration and the run loop (src/funlbm/lbm/base.py:17-227)."""
+
+import logging
+from typing import List
+
+from ..base.work import Worker
+from ..config.base import BaseConfig, FileConfig, deep_get
+from ..flow import FlowConfig, FlowD3, create_flow
+
+logger = logging.getLogger("funlbm_amd")
+
+
+class Config(BaseConfig):
+    """src/funlbm/lbm/base.py:17-40.
+
+    Extension keys of this package (absent from the reference's configs):
+      fused      run_step uses the fused HIP step (default true); false runs
+                 the reference's method sequence, one kernel per method
+      log_every  cadence of the per-step statistics of run() (default 1, the
+                 reference's; 0 disables them)
+    """
+
+    def __init__(self, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+        self.dt: float = 1.0
+        self.dx: float = 1.0
+        self.checkpoint = None
+        self.max_step = 10000
+        self.device: str = "auto"
+        self.file_config = FileConfig()
+        self.flow_config = FlowConfig()
+        self.particles: List = []
+        self.fused: bool = True
+        self.log_every: int = 1
+
+    def _from_json(self, config_json: dict, *args, **kwargs) -> "Config":
+        from ..particle import ParticleConfig
+
+        self.dt = deep_get(config_json, "dt") or self.dt
+        self.dx = deep_get(config_json, "dx") or self.dx
+        self.device = deep_get(config_json, "device") or self.device
+        self.max_step = deep_get(config_json, "max_step") or self.max_step
+        self.checkpoint = deep_get(config_json, "checkpoint") or self.checkpoint
+        self.file_config = FileConfig().from_json(deep_get(config_json, "file") or {})
+        self.flow_config = FlowConfig().from_json(deep_get(config_json, "flow") or {})
+        for config in deep_get(config_json, "particles") or []:
+            self.particles.append(ParticleConfig().from_json(config_json=config))
+        fused = deep_get(config_json, "fused")
+        self.fused = True if fused is None else bool(fused)
+        le = deep_get(config_json, "log_every")
+        self.log_every = 1 if le is None else int(le)
+        return self
+
+
+def create_lbm_config(path="./config.json") -> Config:
+    return Config().from_file(path)
+
+
+class LBMBase(Worker):
+    """Solver object and step orchestration (src/funlbm/lbm/base.py:43-227)."""
+
+    def __init__(self, config: Config = None, flow: FlowD3 = None, particle_swarm=None,
+                 device=None, *args, **kwargs):
+        from ..particle import create_particle_swarm
+
+        self.config: Config = config or create_lbm_config()
+        dev = device
+        if dev is None and self.config.device not in (None, "auto", "cpu"):
+            dev = self.config.device
+        super().__init__(device=dev)
+        self.step = 1
+        self.flow = flow or create_flow(flow_config=self.config.flow_config, device=self.device)
+        fc = self.config.flow_config
+        self.particle_swarm = particle_swarm or create_particle_swarm(
+            self.config.particles, device=self.device, gravity=fc.gravity, gl=fc.gl
+        )
+        self.run_status = True
+        self.history = []
+
+    def run(self, max_steps: int = 1000000, *args, **kwargs) -> None:
+        """src/funlbm/lbm/base.py:86-103."""
+        self.init()
+        total_steps = min(max_steps, self.config.max_step)
+        for _ in range(total_steps):
+            self.run_step(step=self.step)
+            self._log_step_info(self.step)
+            if self.run_status is False:
+                break
+            if self.step >= total_steps:
+                break
+            self.step += 1
+
+    def _log_step_info(self, *args, **kwargs) -> None:
+        """Per-step statistics (src/funlbm/lbm/base.py:105-144) at the
+        configured cadence."""
+        every = self.config.log_every
+        if not every or self.step % every:
+            return
+        self._check_run_status()
+        rec = {"step": self.step, "flow": self.flow.to_json()}
+        rec["particles"] = [p.to_json() for p in self.particle_swarm.particles]
+        self.history.append(rec)
+        logger.info("step=%6d %s", self.step, rec["flow"])
+
+    def _check_run_status(self):
+        pass
+
+    def run_step(self, *args, **kwargs) -> None:
+        """src/funlbm/lbm/base.py:146-157."""
+        if self.config.fused:
+            self._run_step_fused()
+        else:
+            self._compute_flow()
+            self._handle_immersed_boundary()
+            self._update_particles()
+        self.save(self.step)
+
+    def _compute_flow(self) -> None:
+        """src/funlbm/lbm/base.py:159-163."""
+        self.flow.cul_equ(step=self.step)
+        self.flow.f_stream()
+        self.flow.update_u_rou(step=self.step)
+
+    def _handle_immersed_boundary(self) -> None:
+        """src/funlbm/lbm/base.py:165-175."""
+        self.flow_to_lagrange()
+        self.particle_to_wall()
+        for particle in self.particle_swarm.particles:
+            particle.update_from_lar(dt=self.config.dt, gl=self.config.flow_config.gl)
+        self.lagrange_to_flow()
+        self.flow.cul_equ2()
+        self.flow.update_u_rou()
+
+    def _update_particles(self) -> None:
+        """src/funlbm/lbm/base.py:177-179."""
+        self.particle_swarm.update(dt=self.config.dt)
+
+    def _run_step_fused(self) -> None:
+        raise NotImplementedError()
+
+    def init(self, *args, **kwargs) -> None:
+        self._init()
+
+    def _init(self, *args, **kwargs):
+        raise NotImplementedError()
+
+    def flow_to_lagrange(self, *args, **kwargs):
+        raise NotImplementedError()
+
+    def lagrange_to_flow(self, *args, **kwargs):
+        raise NotImplementedError()
+
+    def particle_to_wall(self, *args, **kwargs):
+        raise NotImplementedError()
+
+    def save(self, step=10, *args, **kwargs):
+        """Checkpoint output of the reference (src/funlbm/lbm/base.py:198-203)
+        is not written by this package."""
+        return None

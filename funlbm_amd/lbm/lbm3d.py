@@ -1,0 +1,131 @@
+"""3-D solver with immersed-boundary particles (src/funlbm/lbm/lbm3d.py:11-264).
+
+run_step (fused, the default) issues per step:
+  [particles]  ibm_interp (pull + moments with FOL_old at the marker stencils),
+               wall_spring, particle_forces (double), ibm_spread into FOL_new
+  fused_step   stream, forcing with FOL_new, moments, collide -- one pass
+  [particles]  particle_pose (double rotation, float markers)
+The explicit path (config "fused": false) runs the reference's method
+sequence with one kernel per method; both give the same results.
+"""
+
+import torch
+
+from .. import _native as nat
+from .base import LBMBase
+
+
+class LBMD3(LBMBase):
+    def __init__(self, *args, **kwargs):
+        super().__init__(*args, **kwargs)
+
+    def _init(self):
+        """flow.init, one extra collision, particles (src/funlbm/lbm/lbm3d.py:23-30)."""
+        self.flow.init()
+        self.flow.cul_equ()
+        self.particle_swarm.init()
+        sw = self.particle_swarm
+        self._has_particles = sw.n_markers > 0
+        if self._has_particles:
+            dev = self.device
+            self._owner = torch.zeros(self.flow.n, device=dev, dtype=torch.int64)
+            self._origin = [
+                torch.zeros((sw.n_markers, 3), device=dev, dtype=torch.int32) for _ in range(2)
+            ]
+            self._origin_ok = [False, False]
+
+    # ------------------------------------------------------------------
+    # IBM phases
+    # ------------------------------------------------------------------
+    def _interp(self, from_state: bool):
+        sw, fl = self.particle_swarm, self.flow
+        if from_state:
+            state, rface = fl.state_source()
+            args = (nat.ptr(state), nat.ptr(fl.fol_old()), nat.ptr(rface), None, None)
+        else:
+            args = (None, None, None, nat.ptr(fl._u), nat.ptr(fl._rho))
+        nat.call(
+            "funlbm_ibm_interp", fl.lat if from_state else fl.lat32, *args, sw.n_markers,
+            nat.ptr(sw.lx), nat.ptr(sw.pid), nat.ptr(sw.pstate), float(self.config.dt),
+            1 if self.step > 100 else 0, nat.ptr(sw.lu), nat.ptr(sw.lF), nat.ptr(sw.ctl),
+            fl._stream(),
+        )
+
+    def flow_to_lagrange(self, *args, **kwargs):
+        """Interpolation and direct forcing (src/funlbm/lbm/lbm3d.py:66-100)."""
+        if self._has_particles:
+            self._interp(from_state=False)
+
+    def particle_to_wall(self, *args, **kwargs):
+        """Wall spring (src/funlbm/lbm/lbm3d.py:121-167)."""
+        sw, fl = self.particle_swarm, self.flow
+        if not self._has_particles:
+            return
+        nat.call(
+            "funlbm_wall_spring", fl.lat32, len(sw.particles), nat.ptr(sw.offsets), nat.ptr(sw.lx),
+            nat.ptr(sw.lF), float(self.config.dx), fl._stream(),
+        )
+
+    def _spread(self):
+        sw, fl = self.particle_swarm, self.flow
+        j = 1 - fl._fc
+        buf = fl.fol_next_buffer()
+        if fl._fol_live[j] and not self._origin_ok[j]:
+            buf.zero_()
+        clear = nat.ptr(self._origin[j]) if (fl._fol_live[j] and self._origin_ok[j]) else None
+        nat.call(
+            "funlbm_ibm_spread", fl.lat32, sw.n_markers, nat.ptr(sw.lx), nat.ptr(sw.lF), nat.ptr(sw.lm),
+            nat.ptr(buf), clear, nat.ptr(self._origin[j]), nat.ptr(self._owner), nat.ptr(sw.ctl),
+            fl._stream(),
+        )
+        self._origin_ok[j] = True
+
+    def lagrange_to_flow(self, *args, **kwargs):
+        """Spreading into a zeroed FOL (src/funlbm/lbm/lbm3d.py:102-119)."""
+        if self._has_particles:
+            self._spread()
+        self.flow.publish_fol(self._has_particles)
+
+    def _run_step_fused(self):
+        fl, sw = self.flow, self.particle_swarm
+        fl.begin_step()
+        if self._has_particles:
+            self._interp(from_state=True)
+            self.particle_to_wall()
+            sw.forces(self.config.dt, 1.0)
+            self._spread()
+        fl.finish_step(self._has_particles)
+        if self._has_particles:
+            sw.pose()
+
+    def _handle_immersed_boundary(self) -> None:
+        """Explicit path: the reference's sequence (src/funlbm/lbm/base.py:165-175)."""
+        self.flow_to_lagrange()
+        self.particle_to_wall()
+        if self._has_particles:
+            self.particle_swarm.forces(self.config.dt, 1.0)
+        self.lagrange_to_flow()
+        self.flow.cul_equ2()
+        self.flow.update_u_rou()
+
+    def _check_run_status(self):
+        """The weight-sum failure check of flow_to_lagrange
+        (src/funlbm/lbm/lbm3d.py:84-86), read back at the logging cadence."""
+        if self._has_particles and int(self.particle_swarm.ctl[1].item()) != 0:
+            self.run_status = False
+
+
+class LBMD3Q27(LBMD3):
+    pass
+
+
+class LBMD3Q19(LBMD3):
+    pass
+
+
+class LBMD3Q15(LBMD3):
+    pass
+
+
+class LBMD3Q13(LBMD3):
+    pass

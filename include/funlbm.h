@@ -1,0 +1,206 @@
+/*
+ * funlbm.h - C ABI of the MI355X (gfx950) lattice-Boltzmann + immersed-boundary
+ * timestep library (libfunlbm_hip.so).
+ *
+ * Conventions
+ *   - Every entry point returns int: 0 = ok, > 0 = a hipError_t, < 0 = an
+ *     argument/shape error.  funlbm_last_error() gives a thread-local message.
+ *   - All array arguments are device pointers owned by the caller (PyTorch's
+ *     caching allocator in the Python host).  Nothing is allocated inside a call,
+ *     so every entry point may be recorded into a HIP graph.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).  Calls
+ *     are asynchronous with respect to the host.
+ *   - Populations are stored structure-of-arrays: f[k][x][y][z], z fastest,
+ *     direction order k exactly as the reference lattice tables
+ *     (src/funlbm/flow/d3/d3q13.py:7-44, d3q15.py:7-48, d3q19.py:7-60,
+ *     d3q27.py:7-77).  Vector fields are [3][x][y][z], scalar fields [x][y][z].
+ *     The reference's [X,Y,Z,Q] tensors are the permuted views of these.
+ *   - Node counts are the reference's config size + 1 per axis
+ *     (src/funlbm/flow/d3/base.py:81-82).
+ */
+#ifndef FUNLBM_H
+#define FUNLBM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FUNLBM_ABI_VERSION 1
+
+/* Boundary codes: the values of the reference enum BoundaryCondition
+ * (src/funlbm/config/base.py:22-28). */
+#define FUNLBM_BC_PERIODICAL 11000
+#define FUNLBM_BC_WALL 1200
+#define FUNLBM_BC_WALL_WITH_SPEED 1201
+
+/* Face order of bc[], uw[] and the bits of uzero_mask: the reference's
+ * input(x=0), output(x=X), back(y=0), forward(y=Y), bottom(z=0), top(z=Z)
+ * (src/funlbm/flow/d3/base.py:189-196). */
+
+/* Error codes (negative; positive values are hipError_t). */
+#define FUNLBM_EINVAL -1                /* bad argument or shape */
+#define FUNLBM_EUNSUPPORTED_BC -2       /* boundary code this path does not implement */
+#define FUNLBM_EUNSUPPORTED_LATTICE -3  /* lattice other than D3Q13/15/19/27 */
+#define FUNLBM_ETOOBIG -4               /* size beyond a documented maximum */
+#define FUNLBM_EFAILED -5               /* a device-side check failed */
+
+/* Storage of the population state buffers used by the fused path. */
+#define FUNLBM_STORE_F32 0  /* f as float */
+#define FUNLBM_STORE_F16 1  /* (f - w_k*rho0)*dev_scale as IEEE binary16 */
+#define FUNLBM_STORE_BF16 2 /* (f - w_k*rho0)*dev_scale as bfloat16 */
+#define FUNLBM_STORE_F64 3  /* f as double, arithmetic in double */
+
+typedef struct funlbm_lattice {
+  int32_t q;          /* 13, 15, 19 or 27 */
+  int32_t nx, ny, nz; /* node counts */
+  int32_t storage;    /* FUNLBM_STORE_* */
+  int32_t bc[6];      /* FUNLBM_BC_* per face */
+  int32_t uzero_mask; /* bit i: u := 0 on face i after every moment update (0x3f = reference) */
+  float tau;          /* BGK relaxation time, 3*mu + 0.5 */
+  float rho0;         /* reference density of deviation storage (1.0) */
+  float dev_scale;    /* scale of deviation storage (ignored for F32/F64) */
+  float body_force[3];/* uniform force added to FOL everywhere (extension; 0 = none) */
+  float uw[6][3];     /* wall velocity of WALL_WITH_SPEED faces */
+} funlbm_lattice;
+
+/* ABI version (FUNLBM_ABI_VERSION).  No reference counterpart. */
+int funlbm_abi_version(void);
+
+/* Thread-local description of the last error.  No reference counterpart. */
+const char* funlbm_last_error(void);
+
+/* Bytes of one population buffer (q * nodes * element size) for `lat`. */
+int funlbm_state_bytes(const funlbm_lattice* lat, int64_t* bytes);
+
+/* ---- reference-structured operations (explicit path, float, in place) ---- */
+
+/* BGK collision, replaces FlowD3.cul_equ (src/funlbm/flow/d3/base.py:150-159):
+ * feq = (1 + (u.e)/cs^2 + ((u.e)/cs^2)^2/2 - |u|^2/cs^2/2) * rho*w,
+ * f = f - (f - feq)/tau.  feq may be NULL. */
+int funlbm_collide(const funlbm_lattice* lat, float* f, const float* u, const float* rho,
+                   float* feq, void* stream);
+
+/* Streaming with the six face overrides, replaces FlowD3.f_stream and
+ * f_stream_bound* (src/funlbm/flow/d3/base.py:161-323).  Works on the storage
+ * format of lat->storage, so it is a bit-exact permutation for PERIODICAL and
+ * WALL faces.  rho (float, [x][y][z]) is read only on WALL_WITH_SPEED faces. */
+int funlbm_stream(const funlbm_lattice* lat, const void* f_src, void* f_dst, const float* rho,
+                  void* stream);
+
+/* Moments, replaces FlowD3.update_u_rou + update_u_rou_boundary
+ * (src/funlbm/flow/d3/base.py:116-133): rho = sum f, p = rho/3,
+ * u = (f.e + FOL/2)/rho, then u := 0 on the faces of uzero_mask.
+ * fol may be NULL (zero); body_force is added to fol.  p may be NULL. */
+int funlbm_moments(const funlbm_lattice* lat, const float* f, const float* fol, float* u,
+                   float* rho, float* p, void* stream);
+
+/* Forcing term, replaces FlowD3.cul_equ2 (src/funlbm/flow/d3/base.py:103-114):
+ * f_a += (1 - 1/(2 tau)) w_a sum_d [(e_ad - u_d)/cs^2 + u_d e_ad e_ad/cs^4] FOL_d. */
+int funlbm_forcing(const funlbm_lattice* lat, float* f, const float* u, const float* fol,
+                   void* stream);
+
+/* ---- fused path ---- */
+
+/* Converts float populations (the reference's f after cul_equ) into a state
+ * buffer of lat->storage.  No reference counterpart (storage format). */
+int funlbm_pack(const funlbm_lattice* lat, const float* f, void* state, void* stream);
+
+/* State buffer = equilibrium populations feq(rho, u) computed in the storage's
+ * arithmetic type (rho NULL = 1, u NULL = 0); FlowD3.init's f = rho w is the
+ * u = 0 case (src/funlbm/flow/d3/base.py:94-100). */
+int funlbm_equilibrium_state(const funlbm_lattice* lat, const float* rho, const float* u,
+                             void* state, void* stream);
+
+/* One fused pass replacing, for every node, the tail of LBMBase.run_step
+ * (src/funlbm/lbm/base.py:159-175: f_stream, update_u_rou, cul_equ2,
+ * update_u_rou) and the cul_equ that opens the next step (:161):
+ * pull-stream state_src with the face overrides, where FOL_new != 0 add the
+ * forcing term with u' from FOL_old, take moments with FOL_new, zero u on the
+ * masked faces, collide, store into state_dst.  fol_old / fol_new ([3][x][y][z]
+ * float) may be NULL (zero).  rho_prev / rho_next are used only with
+ * WALL_WITH_SPEED faces (the density of the previous step on those faces). */
+int funlbm_fused_step(const funlbm_lattice* lat, const void* state_src, void* state_dst,
+                      const float* fol_old, const float* fol_new, const float* rho_prev,
+                      float* rho_next, void* stream);
+
+/* The same pass as funlbm_fused_step without the collision: writes the
+ * reference's f, u, rho and p (float) after the step.  Any output may be NULL. */
+int funlbm_materialize(const funlbm_lattice* lat, const void* state_src, const float* fol_old,
+                       const float* fol_new, const float* rho_prev, float* f, float* u,
+                       float* rho, float* p, void* stream);
+
+/* ---- immersed boundary and rigid particles ----
+ *
+ * Markers of all particles are concatenated particle-major (the reference's
+ * loop order, src/funlbm/lbm/lbm3d.py:68,106): lx/lu/lF/lag are float[M][3],
+ * lm float[M], marker_particle int32[M], marker_offsets int32[P+1].
+ * Particle state is double[P][FUNLBM_PSTRIDE] with the offsets below; `mass`
+ * is the volume as in the reference (src/funlbm/particle/base/sphere.py:33-37).
+ * ctl is int32[4] device memory: [0] spreading stamp, [1] weight-sum failure
+ * flag, [2..3] reserved; zero it once before first use. */
+#define FUNLBM_P_CX 0       /* centre [3] */
+#define FUNLBM_P_CU 3       /* velocity [3] */
+#define FUNLBM_P_CW 6       /* angular velocity [3] */
+#define FUNLBM_P_CF 9       /* force [3] */
+#define FUNLBM_P_CT 12      /* torque [3] */
+#define FUNLBM_P_ANGLE 15   /* accumulated rotation vector [3] */
+#define FUNLBM_P_ROT 18     /* rotation matrix of angle, row-major [9] */
+#define FUNLBM_P_MASS 27    /* volume */
+#define FUNLBM_P_INERTIA 28 /* principal moments [3] */
+#define FUNLBM_P_RHO 31     /* particle density */
+#define FUNLBM_P_GRAV 32    /* gravity vector [3]; the reference's is (gl, 0, 0) */
+#define FUNLBM_PSTRIDE 40
+
+/* Interpolation and direct forcing, replaces LBMD3.flow_to_lagrange
+ * (src/funlbm/lbm/lbm3d.py:46-100): per marker, cosine-kernel weights
+ * (1 + cos(pi r/2))/4 per axis on the stencil [max(floor(x)-1,0), min(+4,size)),
+ * normalised to sum 1; s_u = sum w (ux + uy + uz); s_rho = sum w rho;
+ * lu = cu + cw x (lx - cx); lF = 2 s_rho (lu - s_u)/dt.  u and rho come from
+ * the fused state (state != NULL: pull + moments with fol_old) or from the
+ * explicit fields u/rho.  check_weight_sum != 0 sets ctl[1] for a marker whose
+ * weights do not sum to 1 within 1e-3 (the reference checks after step 100). */
+int funlbm_ibm_interp(const funlbm_lattice* lat, const void* state, const float* fol_old,
+                      const float* rho_face, const float* u, const float* rho, int32_t n_markers,
+                      const float* lx, const int32_t* marker_particle, const double* pstate,
+                      float dt, int32_t check_weight_sum, float* lu, float* lF, int32_t* ctl,
+                      void* stream);
+
+/* Wall spring, replaces LBMD3.particle_to_wall (src/funlbm/lbm/lbm3d.py:121-167). */
+int funlbm_wall_spring(const funlbm_lattice* lat, int32_t n_particles, const int32_t* marker_offsets,
+                       const float* lx, float* lF, float dx, void* stream);
+
+/* Rigid-body update in double, replaces Particle.update_from_lar
+ * (src/funlbm/particle/base/base.py:120-151).  rho_fluid is the reference's
+ * rouf (1.0); dt <= 0 or rho_fluid <= 0 is FUNLBM_EINVAL. */
+int funlbm_particle_forces(int32_t n_particles, const int32_t* marker_offsets, const float* lx,
+                           const float* lF, const float* lm, double* pstate, double dt,
+                           double rho_fluid, void* stream);
+
+/* Pose update, replaces Particle.update + Coordinate.update/cul_point
+ * (src/funlbm/particle/base/base.py:153-156, src/funlbm/particle/coord.py:73-108):
+ * if advance, angle += cw; R = rotation of the rotation vector angle;
+ * lx = float(R lag) + float(cx). */
+int funlbm_particle_pose(int32_t n_particles, const int32_t* marker_offsets, const float* lag,
+                         float* lx, double* pstate, int32_t advance, void* stream);
+
+/* Spreading, replaces LBMD3.lagrange_to_flow (src/funlbm/lbm/lbm3d.py:102-119):
+ * fol := 0 and then, for every marker in order, fol[stencil] = w lF lm
+ * overwriting.  The buffer must be zero outside the stencils whose origins
+ * (int32[M][3]) are given in clear_origin (NULL: zero everywhere); the origins
+ * of this call are written to `origin`.  owner is uint64[nodes] device memory,
+ * zeroed once before first use and never again. */
+int funlbm_ibm_spread(const funlbm_lattice* lat, int32_t n_markers, const float* lx, const float* lF,
+                      const float* lm, float* fol, const int32_t* clear_origin, int32_t* origin,
+                      uint64_t* owner, int32_t* ctl, void* stream);
+
+/* Plain device copy of n_bytes (16-byte aligned), used to calibrate the
+ * measured copy bandwidth.  No reference counterpart. */
+int funlbm_copy_calibrate(const void* src, void* dst, int64_t n_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FUNLBM_H */

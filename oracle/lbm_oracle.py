@@ -1,0 +1,297 @@
+"""CPU oracle of the funlbm per-timestep fluid path.  TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py may
+import this module, and only as the checker; the product package funlbm_amd
+never does.
+
+It restates, in numpy on the reference's own [X, Y, Z, Q] layout, the operation
+sequence of the reference solver:
+
+  init                 src/funlbm/flow/d3/base.py:72-101, lbm/lbm3d.py:23-30
+  cul_equ  (collide)   src/funlbm/flow/d3/base.py:150-159
+  f_stream (stream)    src/funlbm/flow/d3/base.py:161-323 (torch.roll, then the
+                       six face overrides in the reference's order)
+  update_u_rou         src/funlbm/flow/d3/base.py:116-133
+  cul_equ2 (forcing)   src/funlbm/flow/d3/base.py:103-114
+  step order           src/funlbm/lbm/base.py:146-179
+
+Arithmetic runs in the array dtype (float32 like the reference, or float64).
+Where torch uses a BLAS product (u @ e^T, f @ e) or a library reduction
+(sum over Q, the 3-norm), this restatement sums in direction order instead;
+the two differ by rounding only.  In float64 the constants are cs^2 = 1/3 and
+cs^4 = 1/9 exactly.
+
+Two extensions that the reference lacks are restated the way the product
+defines them: a uniform body force g added to FOL everywhere, and a mask of
+the faces on which u is zeroed (the reference zeroes all six, mask 0x3f).
+"""
+
+import math
+
+import numpy as np
+
+# Lattice tables, in the reference's direction order
+# (src/funlbm/flow/d3/d3q13.py:7-44, d3q15.py:7-48, d3q19.py:7-60, d3q27.py:7-77).
+_Q19_E = [
+    [0, 0, 0],
+    [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1],
+    [1, 1, 0], [-1, 1, 0], [1, -1, 0], [-1, -1, 0],
+    [0, 1, 1], [0, -1, 1], [0, 1, -1], [0, -1, -1],
+    [1, 0, 1], [-1, 0, 1], [1, 0, -1], [-1, 0, -1],
+]
+LATTICE_E = {
+    13: [
+        [0, 0, 0], [1, 1, 0], [1, -1, 0], [1, 0, 1], [1, 0, -1], [0, 1, 1], [0, 1, -1],
+        [-1, -1, 0], [-1, 1, 0], [-1, 0, -1], [-1, 0, 1], [0, -1, -1], [0, -1, 1],
+    ],
+    15: [
+        [0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1],
+        [1, 1, 1], [-1, 1, 1], [1, -1, 1], [-1, -1, 1],
+        [1, 1, -1], [-1, 1, -1], [1, -1, -1], [-1, -1, -1],
+    ],
+    19: _Q19_E,
+    27: _Q19_E + [
+        [1, 1, 1], [1, 1, -1], [1, -1, 1], [1, -1, -1],
+        [-1, 1, 1], [-1, 1, -1], [-1, -1, 1], [-1, -1, -1],
+    ],
+}
+LATTICE_W = {
+    13: [1.0 / 2] + [1.0 / 24] * 12,
+    15: [2.0 / 9] + [1.0 / 9] * 6 + [1.0 / 72] * 8,
+    19: [1.0 / 3] + [1.0 / 18] * 6 + [1.0 / 36] * 12,
+    27: [8.0 / 27] + [2.0 / 27] * 6 + [1.0 / 54] * 12 + [1.0 / 216] * 8,
+}
+
+PERIODICAL = 11000
+WALL = 1200
+WALL_WITH_SPEED = 1201
+NON_EQUILIBRIUM = 1400
+
+FACES = ("input", "output", "back", "forward", "bottom", "top")
+
+
+class Lattice:
+    """Param (src/funlbm/flow/base.py:15-75) for one of D3Q13/15/19/27."""
+
+    def __init__(self, q: int, dtype=np.float32):
+        if q not in LATTICE_E:
+            raise ValueError(f"Unknown parameter type: D3Q{q}")
+        self.q = q
+        self.dtype = np.dtype(dtype)
+        self.e = np.array(LATTICE_E[q], dtype=np.int64)
+        self.w = np.array(LATTICE_W[q], dtype=np.float64).astype(self.dtype)
+        # reverse map through the string key of -e, as d3q19.py:55-60
+        key = {",".join(str(i) for i in xyz): i for i, xyz in enumerate(LATTICE_E[q])}
+        self.rev = [key[",".join(str(int(-1 * i)) for i in xyz)] for xyz in LATTICE_E[q]]
+        if self.dtype == np.float32:
+            cs = np.float32(math.sqrt(1.0 / 3))
+            self.cs2 = np.float32(cs * cs)
+            self.cs4 = np.float32(self.cs2 * self.cs2)
+        else:
+            self.cs2 = self.dtype.type(1.0 / 3.0)
+            self.cs4 = self.dtype.type(1.0 / 9.0)
+
+    def vertex_index(self, axis: int, value: int):
+        """src/funlbm/flow/base.py:46-57."""
+        return [i for i, e in enumerate(self.e) if e[axis] == value]
+
+
+class FlowState:
+    """The reference Flow's tensors (src/funlbm/flow/base.py:141-157), AoS."""
+
+    def __init__(self, lat: Lattice, nodes, tau, bc, uw=None, uzero_mask=0x3F, body_force=None):
+        self.lat = lat
+        self.dt = lat.dtype
+        self.nodes = tuple(int(n) for n in nodes)
+        self.tau = self.dt.type(tau)
+        self.bc = list(bc)
+        self.uw = [np.zeros(3) if uw is None or uw[i] is None else np.asarray(uw[i]) for i in range(6)]
+        self.uzero_mask = int(uzero_mask)
+        self.body = None if body_force is None else np.asarray(body_force, dtype=self.dt)
+        shape = self.nodes
+        self.f = np.zeros(shape + (lat.q,), dtype=self.dt)
+        self.feq = np.zeros_like(self.f)
+        self.u = np.zeros(shape + (3,), dtype=self.dt)
+        self.rou = np.ones(shape + (1,), dtype=self.dt)
+        self.p = np.zeros(shape + (1,), dtype=self.dt)
+        self.FOL = np.zeros(shape + (3,), dtype=self.dt)
+
+    def fol_eff(self):
+        if self.body is None:
+            return self.FOL
+        return self.FOL + self.body
+
+
+def init_flow(q, nodes, mu=None, tau=None, bc=(WALL,) * 6, dtype=np.float32, uw=None,
+              uzero_mask=0x3F, body_force=None):
+    """FlowD3.init (src/funlbm/flow/d3/base.py:72-101): u = 0, rho = 1,
+    tau = 3 mu + 0.5, FOL = 0, f = rho w, then update_u_rou.  The extra
+    cul_equ of LBMD3._init (src/funlbm/lbm/lbm3d.py:29) is init_solver's."""
+    lat = Lattice(q, dtype)
+    if tau is None:
+        tau = np.float32(3 * mu + 0.5) if lat.dtype == np.float32 else 3 * mu + 0.5
+    st = FlowState(lat, nodes, tau, bc, uw=uw, uzero_mask=uzero_mask, body_force=body_force)
+    st.f = (st.rou * lat.w[None, None, None, :]).astype(st.dt)
+    update_u_rou(st)
+    return st
+
+
+def face_mask(nodes, uzero_mask):
+    """Boolean [X,Y,Z] of the nodes on the masked faces."""
+    m = np.zeros(nodes, dtype=bool)
+    if uzero_mask & 1:
+        m[0, :, :] = True
+    if uzero_mask & 2:
+        m[-1, :, :] = True
+    if uzero_mask & 4:
+        m[:, 0, :] = True
+    if uzero_mask & 8:
+        m[:, -1, :] = True
+    if uzero_mask & 16:
+        m[:, :, 0] = True
+    if uzero_mask & 32:
+        m[:, :, -1] = True
+    return m
+
+
+def moments(lat, f, F, uzero):
+    """rho = sum_k f_k; u = (f.e + F/2)/rho; u := 0 where uzero."""
+    dt = f.dtype
+    r = np.zeros(f.shape[:-1], dtype=dt)
+    j = [np.zeros(f.shape[:-1], dtype=dt) for _ in range(3)]
+    for k in range(lat.q):
+        r = r + f[..., k]
+        for d in range(3):
+            if lat.e[k, d] == 1:
+                j[d] = j[d] + f[..., k]
+            elif lat.e[k, d] == -1:
+                j[d] = j[d] - f[..., k]
+    u = np.stack([(j[d] + F[..., d] / dt.type(2)) / r for d in range(3)], axis=-1)
+    u[uzero] = 0
+    return r[..., None], u
+
+
+def update_u_rou(st: FlowState):
+    """src/funlbm/flow/d3/base.py:116-133 (PERIODICAL/WALL faces)."""
+    rho, u = moments(st.lat, st.f, st.fol_eff(), face_mask(st.nodes, st.uzero_mask))
+    st.rou = rho
+    st.p = rho / st.dt.type(3.0)
+    st.u = u
+
+
+def _edotu(lat, k, u):
+    dt = u.dtype
+    d = np.zeros(u.shape[:-1], dtype=dt)
+    for a in range(3):
+        if lat.e[k, a] == 1:
+            d = d + u[..., a]
+        elif lat.e[k, a] == -1:
+            d = d - u[..., a]
+    return d
+
+
+def equilibrium(lat, rho, u):
+    """feq_k = (1 + tmp + tmp^2/2 - u2)(rho w_k), tmp = (u.e_k)/cs^2,
+    u2 = |u|^2/cs^2/2 (src/funlbm/flow/d3/base.py:152-158)."""
+    dt = u.dtype
+    nrm = np.sqrt(u[..., 0] * u[..., 0] + u[..., 1] * u[..., 1] + u[..., 2] * u[..., 2])
+    u2 = nrm * nrm / lat.cs2 / dt.type(2)
+    feq = np.empty(u.shape[:-1] + (lat.q,), dtype=dt)
+    for k in range(lat.q):
+        tmp = _edotu(lat, k, u) / lat.cs2
+        feq[..., k] = ((dt.type(1) + tmp) + tmp * tmp / dt.type(2) - u2) * (rho[..., 0] * lat.w[k])
+    return feq
+
+
+def cul_equ(st: FlowState):
+    """BGK collision (src/funlbm/flow/d3/base.py:150-159)."""
+    st.feq = equilibrium(st.lat, st.rou, st.u)
+    st.f = st.f - (st.f - st.feq) / st.tau
+
+
+def f_stream(st: FlowState):
+    """torch.roll streaming and face overrides, literally as
+    src/funlbm/flow/d3/base.py:161-323."""
+    lat = st.lat
+    fcopy = st.f.copy()
+    f = st.f.copy()
+    for k in range(1, lat.q):
+        t = fcopy[..., k]
+        for axis in range(3):
+            s = int(lat.e[k, axis])
+            if s != 0:
+                t = np.roll(t, shift=s, axis=axis)
+        f[..., k] = t
+    # input, output, back, forward, bottom, top (src/funlbm/flow/d3/base.py:189-196)
+    sl_lo = slice(0, 1)
+    sl_hi = slice(-1, None)
+    faces = [
+        (0, 1, sl_lo, sl_hi),   # input:  x=0, e_x=+1, periodic source x=X
+        (0, -1, sl_hi, sl_lo),  # output
+        (1, 1, sl_lo, sl_hi),   # back
+        (1, -1, sl_hi, sl_lo),  # forward
+        (2, 1, sl_lo, sl_hi),   # bottom
+        (2, -1, sl_hi, sl_lo),  # top
+    ]
+    for face, (axis, sign, here, there) in enumerate(faces):
+        index = lat.vertex_index(axis, sign)
+        rev = [lat.rev[i] for i in index]
+        bc = st.bc[face]
+
+        def at(sl, arr):
+            key = [slice(None)] * 3
+            key[axis] = sl
+            return arr[tuple(key)]
+
+        if bc == PERIODICAL:
+            at(here, f)[..., index] = at(there, fcopy)[..., index]
+        elif bc == WALL:
+            at(here, f)[..., index] = at(here, fcopy)[..., rev]
+        elif bc == WALL_WITH_SPEED:
+            dt = st.dt
+            uw = np.asarray(st.uw[face], dtype=dt)
+            rho = at(here, st.rou)  # [.., 1]
+            eu = np.zeros(len(index), dtype=dt)
+            for a in range(3):
+                eu = eu + lat.e[index, a].astype(dt) * uw[a]
+            tmp = dt.type(2) * (rho * lat.w[index])
+            tmp = tmp * eu / lat.cs2
+            at(here, f)[..., index] = at(here, fcopy)[..., rev] - tmp
+        else:
+            raise NotImplementedError(f"boundary {bc} on face {FACES[face]}")
+    st.f = f
+
+
+def cul_equ2(st: FlowState):
+    """Forcing term (src/funlbm/flow/d3/base.py:103-114) with FOL (+ body force)."""
+    lat = st.lat
+    dt = st.dt
+    F = st.fol_eff()
+    omega = dt.type(1) - dt.type(1) / (dt.type(2) * st.tau)
+    for a in range(lat.q):
+        t3 = None
+        for d in range(3):
+            e = dt.type(lat.e[a, d])
+            t1 = (e - st.u[..., d]) / lat.cs2
+            t2 = ((st.u[..., d] * e) * e) / lat.cs4
+            term = (t1 + t2) * F[..., d]
+            t3 = term if t3 is None else t3 + term
+        st.f[..., a] = st.f[..., a] + (omega * lat.w[a]) * t3
+
+
+def init_solver(st: FlowState):
+    """LBMD3._init: flow.init() is init_flow; then one extra cul_equ
+    (src/funlbm/lbm/lbm3d.py:23-30)."""
+    cul_equ(st)
+
+
+def fluid_step(st: FlowState, fol_new=None):
+    """One LBMBase.run_step without particles (src/funlbm/lbm/base.py:146-179):
+    collide, stream, moments, FOL := fol_new (zero: lagrange_to_flow with no
+    particles), forcing, moments."""
+    cul_equ(st)
+    f_stream(st)
+    update_u_rou(st)
+    st.FOL = np.zeros_like(st.FOL) if fol_new is None else fol_new.astype(st.dt)
+    cul_equ2(st)
+    update_u_rou(st)

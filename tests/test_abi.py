@@ -1,0 +1,65 @@
+"""The C-ABI library loads and exports every entry point include/funlbm.h
+declares (no compute call; runs without a GPU)."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from funlbm_amd import _native as nat
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "funlbm.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(funlbm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert "funlbm_fused_step" in names and "funlbm_ibm_spread" in names
+    assert len(names) >= 15
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(nat.library_path())
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert missing == []
+
+
+def test_bindings_cover_header():
+    assert sorted(nat.SIGNATURES) == declared_functions()
+
+
+def test_abi_version_and_struct_size():
+    lib = nat.lib()
+    assert lib.funlbm_abi_version() == 1
+    # 5 ints + 6 ints + 1 int + 3 floats + 3 floats + 18 floats
+    assert ctypes.sizeof(nat.Lattice) == 4 * (5 + 6 + 1 + 3 + 3 + 18)
+
+
+def test_argument_errors_need_no_device():
+    lat = nat.Lattice()
+    lat.q = 9
+    lat.nx = lat.ny = lat.nz = 4
+    lat.tau = 0.8
+    rc = nat.lib().funlbm_state_bytes(ctypes.byref(lat), ctypes.byref(ctypes.c_int64()))
+    assert rc == nat.EUNSUPPORTED_LATTICE
+    assert "D3Q9" in nat.last_error()
+    with pytest.raises(ValueError):
+        nat.check(rc, "state_bytes")
+    lat.q = 19
+    for i in range(6):
+        lat.bc[i] = 1400  # NON_EQUILIBRIUM is not implemented by the HIP path
+    rc = nat.lib().funlbm_state_bytes(ctypes.byref(lat), ctypes.byref(ctypes.c_int64()))
+    assert rc == nat.EUNSUPPORTED_BC
+    with pytest.raises(NotImplementedError):
+        nat.check(rc, "state_bytes")
+    for i in range(6):
+        lat.bc[i] = nat.BC_WALL
+    nbytes = ctypes.c_int64()
+    assert nat.lib().funlbm_state_bytes(ctypes.byref(lat), ctypes.byref(nbytes)) == 0
+    assert nbytes.value == 19 * 64 * 4

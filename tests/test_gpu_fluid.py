@@ -1,0 +1,164 @@
+"""Parity of the HIP fluid kernels with the CPU oracle (needs a GPU).
+
+Streaming and bounce-back are permutations and must be bit-exact; the float
+arithmetic must agree with the oracle within 1e-5 relative L-inf after the
+step counts below (the reference's own criterion for 1000 steps)."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import lbm_oracle as orc
+from tests.helpers import flow_json, random_state, rel_linf
+
+pytestmark = pytest.mark.gpu
+
+P, W, MW = orc.PERIODICAL, orc.WALL, orc.WALL_WITH_SPEED
+BC_SETS = {
+    "periodic": [P] * 6,
+    "wall": [W] * 6,
+    "channel": [W, W, P, P, P, P],
+    "mixed": [P, W, W, P, W, P],
+}
+
+
+def make_flow(q, size, mu, bcs, uw=None, **ext):
+    from funlbm_amd.flow import FlowConfig, create_flow
+
+    cfg = FlowConfig().from_json(flow_json(q, size, mu, bcs, uw=uw, **ext))
+    fl = create_flow(cfg, device="cuda")
+    fl.init()
+    return fl
+
+
+def oracle_state(q, size, mu, bcs, uw=None, dtype=np.float32, **kw):
+    nodes = tuple(int(s) + 1 for s in size)
+    return orc.init_flow(q, nodes, mu=mu, bc=bcs, dtype=dtype, uw=uw, **kw)
+
+
+@pytest.mark.parametrize("q", [13, 15, 19, 27])
+@pytest.mark.parametrize("bcname", list(BC_SETS))
+def test_stream_bit_exact(q, bcname):
+    size = (4, 5, 6)
+    bcs = BC_SETS[bcname]
+    fl = make_flow(q, size, 0.1, bcs)
+    st = oracle_state(q, size, 0.1, bcs)
+    f0 = random_state(q, st.nodes, seed=q * 7 + len(bcname))
+    # unique values make every misplaced population visible
+    f0 = f0 + np.arange(f0.size, dtype=np.float32).reshape(f0.shape) * np.float32(1e-3)
+    fl.f = torch.from_numpy(f0)
+    st.f = f0.copy()
+    fl.f_stream()
+    orc.f_stream(st)
+    got = fl.f.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), st.f.view(np.uint32))
+
+
+@pytest.mark.parametrize("q", [15, 19])
+def test_stream_moving_wall(q):
+    size = (5, 4, 6)
+    uw = [None, None, None, None, None, [0.05, 0.01, 0.0]]
+    bcs = [W, W, P, P, W, MW]
+    fl = make_flow(q, size, 0.1, bcs, uw=uw)
+    st = oracle_state(q, size, 0.1, bcs, uw=uw)
+    f0 = random_state(q, st.nodes, seed=3)
+    fl.f = torch.from_numpy(f0)
+    st.f = f0.copy()
+    fl.update_u_rou()
+    orc.update_u_rou(st)
+    fl.f_stream()
+    orc.f_stream(st)
+    assert rel_linf(fl.f.cpu().numpy(), st.f) < 1e-6
+
+
+@pytest.mark.parametrize("q", [13, 15, 19, 27])
+def test_collide_moments_forcing(q):
+    size = (5, 6, 7)
+    bcs = BC_SETS["mixed"]
+    fl = make_flow(q, size, 0.05, bcs)
+    st = oracle_state(q, size, 0.05, bcs)
+    f0 = random_state(q, st.nodes, seed=11 + q)
+    rng = np.random.default_rng(q)
+    fol = (1e-3 * rng.standard_normal(st.nodes + (3,))).astype(np.float32)
+    fl.f = torch.from_numpy(f0)
+    fl.FOL = torch.from_numpy(fol)
+    st.f = f0.copy()
+    st.FOL = fol.copy()
+    fl.update_u_rou()
+    orc.update_u_rou(st)
+    assert rel_linf(fl.u.cpu().numpy(), st.u) < 1e-6
+    assert rel_linf(fl.rou.cpu().numpy(), st.rou) < 1e-6
+    fl.cul_equ()
+    orc.cul_equ(st)
+    assert rel_linf(fl.feq.cpu().numpy(), st.feq) < 1e-6
+    assert rel_linf(fl.f.cpu().numpy(), st.f) < 1e-6
+    fl.cul_equ2()
+    orc.cul_equ2(st)
+    assert rel_linf(fl.f.cpu().numpy(), st.f) < 1e-6
+
+
+def run_solver(q, size, mu, bcs, steps, f0=None, fused=True, **flow_ext):
+    from funlbm_amd.lbm import LBMD3, Config
+
+    cfg = Config().from_json({"flow": flow_json(q, size, mu, bcs, **flow_ext), "log_every": 0,
+                              "fused": fused})
+    lbm = LBMD3(config=cfg)
+    lbm.init()
+    if f0 is not None:
+        lbm.flow.f = torch.from_numpy(f0)
+        lbm.flow.update_u_rou()
+    for s in range(1, steps + 1):
+        lbm.step = s
+        lbm.run_step()
+    return lbm
+
+
+def run_oracle(q, size, mu, bcs, steps, f0=None, body_force=None):
+    nodes = tuple(int(s) + 1 for s in size)
+    st = orc.init_flow(q, nodes, mu=mu, bc=bcs, body_force=body_force)
+    orc.init_solver(st)
+    if f0 is not None:
+        st.f = f0.copy()
+        orc.update_u_rou(st)
+    for _ in range(steps):
+        orc.fluid_step(st)
+    return st
+
+
+@pytest.mark.parametrize("q", [13, 15, 19, 27])
+@pytest.mark.parametrize("bcname", ["periodic", "channel", "wall"])
+def test_fused_steps_match_oracle(q, bcname):
+    size = (6, 7, 9)
+    bcs = BC_SETS[bcname]
+    nodes = tuple(s + 1 for s in size)
+    f0 = random_state(q, nodes, seed=5)
+    lbm = run_solver(q, size, 0.05, bcs, 50, f0=f0)
+    st = run_oracle(q, size, 0.05, bcs, 50, f0=f0)
+    assert rel_linf(lbm.flow.f.cpu().numpy(), st.f) < 1e-5
+    assert rel_linf(lbm.flow.rou.cpu().numpy(), st.rou) < 1e-5
+    assert rel_linf(lbm.flow.u.cpu().numpy(), st.u) < 1e-4
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_fused_equals_explicit_bitwise(q):
+    size = (5, 6, 7)
+    bcs = BC_SETS["mixed"]
+    nodes = tuple(s + 1 for s in size)
+    f0 = random_state(q, nodes, seed=9)
+    a = run_solver(q, size, 0.05, bcs, 20, f0=f0, fused=True)
+    b = run_solver(q, size, 0.05, bcs, 20, f0=f0, fused=False)
+    assert torch.equal(a.flow.f, b.flow.f)
+    assert torch.equal(a.flow.u, b.flow.u)
+
+
+def test_poiseuille_config1_1000_steps():
+    """Channel of the Poiseuille case: 32 x 32 x 64 nodes, walls at x = 0 and
+    x = 31, periodic y and z, tau = 0.8, body force g_z = 1e-6 (FOL extension)."""
+    size = (31, 31, 63)
+    bcs = BC_SETS["channel"]
+    mu = 0.1
+    g = [0.0, 0.0, 1e-6]
+    lbm = run_solver(19, size, mu, bcs, 1000, body_force=g)
+    st = run_oracle(19, size, mu, bcs, 1000, body_force=g)
+    assert rel_linf(lbm.flow.f.cpu().numpy(), st.f) < 1e-5
+    assert rel_linf(lbm.flow.u.cpu().numpy(), st.u) < 1e-4
