@@ -132,6 +132,111 @@ def cpu_baseline(n=64, q=19, budget_s=15.0):
     }
 
 
+def bench_batch(n_cases=256, steps=3000, seed=0, storage="f16", tick=50, baseline_cases=3,
+                baseline_steps=200):
+    """Batched independent settling cases (all cases advance in one persistent
+    launch sequence per step, replayed from a HIP graph) against plain
+    one-case-at-a-time solves of the same cases with LBMD3."""
+    import torch
+
+    from funlbm_amd.batch.engine import BatchEngine, markers_for
+    from funlbm_amd.batch.workloads import settling_batch
+    from funlbm_amd.lbm import LBMD3, Config
+
+    specs = settling_batch(n_cases=n_cases, seed=seed, steps=steps)
+    nodes = sum(s.n for s in specs)
+    markers = sum(len(markers_for(p)[0]) for s in specs for p in s.particles)
+    eng = BatchEngine(storage=storage, max_nodes=nodes + 256 * n_cases, max_markers=markers + 64,
+                      max_particles=n_cases, max_cases=n_cases, max_q=19, traj_cap=tick)
+    for s in specs:
+        eng.add(s)
+    tune = eng.autotune(steps=tick) if hasattr(eng, "autotune") else None
+    eng.capture(tick)
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(-(-steps // tick)):
+        eng.run_graph()
+    b.record()
+    torch.cuda.synchronize()
+    t = a.elapsed_time(b) * 1e-3
+    assert all(eng.left()[c] == 0 for c in eng.slots)
+    lups = sum(s.n * s.steps for s in specs)
+    # one case at a time: the single-lattice solver, eager, same storage
+    per_case = []
+    for s in specs[:baseline_cases]:
+        p = s.particles[0]
+        faces = {f: {"code": "WALL"} for f in ("input", "output", "back", "forward", "bottom", "top")}
+        cfg = Config().from_json({
+            "flow": {"param_type": f"D3Q{s.q}", "size": [n - 1 for n in s.nodes],
+                     "param": {"mu": (s.tau - 0.5) / 3, "gravity": list(s.gravity)},
+                     "boundary": faces, "storage": storage},
+            "particles": [{"type": "sphere", "r": p.r, "rou": p.rou, "coord": {"center": list(p.center)}}],
+            "log_every": 0,
+        })
+        lbm = LBMD3(config=cfg)
+        lbm.init()
+        for k in range(1, 11):
+            lbm.step = k
+            lbm.run_step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(11, 11 + baseline_steps):
+            lbm.step = k
+            lbm.run_step()
+        torch.cuda.synchronize()
+        per_case.append((time.perf_counter() - t0) / baseline_steps * s.steps)
+        del lbm
+    base_solves = 1.0 / (sum(per_case) / len(per_case))
+    solves = n_cases / t
+    return {
+        "workload": f"{n_cases} settling cases, sides log-uniform 32-96, D3Q15/19, one sphere each, "
+                    f"{steps} steps, {storage}-deviation storage",
+        "cases": n_cases,
+        "nodes": nodes,
+        "markers": markers,
+        "seconds": round(t, 3),
+        "solves_per_s": round(solves, 2),
+        "mlups": round(lups / t / 1e6, 1),
+        "one_at_a_time_solves_per_s": round(base_solves, 3),
+        "one_at_a_time_sample": f"{baseline_cases} cases x {baseline_steps} steps with LBMD3, scaled to {steps} steps",
+        "speedup_vs_one_at_a_time": round(solves / base_solves, 2),
+        "grid": dict(eng.grid),
+        "autotune": tune,
+    }
+
+
+def bench_serve(rate=20.0, duration=10.0, seed=0, steps=(1000, 20000), budget=200e9, storage="f16",
+                tick=50, pool_nodes=None):
+    """Poisson stream of settling requests served with continuous batching."""
+    import torch
+
+    from funlbm_amd.batch.engine import BatchEngine, markers_for
+    from funlbm_amd.batch.workloads import serving_requests
+    from funlbm_amd.serve import Request, Server
+
+    reqs = serving_requests(rate, duration, seed=seed, steps=steps)
+    need_nodes = sum(s.n for _, s in reqs)
+    pool_nodes = pool_nodes or min(need_nodes + 256 * len(reqs), 1 << 30)
+    markers = sum(len(markers_for(p)[0]) for _, s in reqs for p in s.particles)
+    eng = BatchEngine(storage=storage, max_nodes=pool_nodes, max_markers=markers + 64,
+                      max_particles=4 * len(reqs) + 4, max_cases=len(reqs) + 1, max_q=27, traj_cap=tick)
+    srv = Server(eng, hbm_budget=budget, tick_steps=tick)
+    stats = srv.run([Request(rid=i, spec=s, arrival=t) for i, (t, s) in enumerate(reqs)])
+    stats.update({
+        "workload": f"Poisson {rate} req/s for {duration} s, sides 32-128, D3Q19/27, 1-4 particles, "
+                    f"steps {steps[0]}-{steps[1]}, {storage}-deviation storage, {budget / 1e9:.0f} GB budget",
+        "rate": rate,
+    })
+    for k in ("wall_s", "busy_s", "solves_per_s", "mlups", "p50_latency_s", "p99_latency_s"):
+        if stats.get(k) is not None:
+            stats[k] = round(stats[k], 3)
+    del srv, eng
+    torch.cuda.empty_cache()
+    return stats
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +247,14 @@ def main():
     ap.add_argument("--storage", default="f32")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-copy", action="store_true")
+    ap.add_argument("--batch-cases", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=3000)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-steps", default="100,2000", help="min,max steps of a request")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-f16", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -193,6 +306,31 @@ def main():
     }
     del lbm
     torch.cuda.empty_cache()
+    if not args.no_f16 and args.storage == "f32":
+        lbm16 = build_solver(args.size, args.q, "f16")
+        ms16, _ = time_steps(lbm16, args.steps, args.warmup)
+        del lbm16
+        torch.cuda.empty_cache()
+        m16 = n / (ms16 * 1e-3) / 1e6
+        out["f16_deviation"] = {
+            "mlups": round(m16, 1),
+            "ms_per_step": round(ms16, 4),
+            "speedup_vs_f32": round(m16 / mlups, 3),
+            "achieved_gbs": round(m16 * 1e6 * 2 * args.q * 2 / 1e9, 1),
+        }
+    if not args.no_batch:
+        try:
+            out["batched"] = bench_batch(n_cases=args.batch_cases, steps=args.batch_steps)
+        except Exception as e:  # reported, never silent
+            out["batched"] = {"error": repr(e)}
+        torch.cuda.empty_cache()
+    if not args.no_serve:
+        lo, hi = (int(v) for v in args.serve_steps.split(","))
+        try:
+            out["serving"] = bench_serve(rate=args.serve_rate, duration=args.serve_seconds, steps=(lo, hi))
+        except Exception as e:
+            out["serving"] = {"error": repr(e)}
+        torch.cuda.empty_cache()
     if not args.no_copy:
         cb = copy_bandwidth()
         out["roofline"]["copy_peak_gbs"] = round(cb, 1)

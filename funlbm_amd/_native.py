@@ -79,8 +79,8 @@ SIGNATURES = {
     "funlbm_forcing": (c_int32, [LATP, P, P, P, P]),
     "funlbm_pack": (c_int32, [LATP, P, P, P]),
     "funlbm_equilibrium_state": (c_int32, [LATP, P, P, P, P]),
-    "funlbm_fused_step": (c_int32, [LATP, P, P, P, P, P, P, P]),
-    "funlbm_materialize": (c_int32, [LATP, P, P, P, P, P, P, P, P, P]),
+    "funlbm_fused_step": (c_int32, [LATP, P, P, P, P, P, P, P, P, P]),
+    "funlbm_materialize": (c_int32, [LATP, P, P, P, P, P, P, P, P, P, P, P]),
     "funlbm_copy_calibrate": (c_int32, [P, P, c_int64, P]),
     "funlbm_ibm_interp": (
         c_int32,
@@ -89,7 +89,10 @@ SIGNATURES = {
     "funlbm_wall_spring": (c_int32, [LATP, c_int32, P, P, P, c_float, P]),
     "funlbm_particle_forces": (c_int32, [c_int32, P, P, P, P, P, ctypes.c_double, ctypes.c_double, P]),
     "funlbm_particle_pose": (c_int32, [c_int32, P, P, P, P, c_int32, P]),
-    "funlbm_ibm_spread": (c_int32, [LATP, c_int32, P, P, P, P, P, P, P, P, P]),
+    "funlbm_ibm_spread": (c_int32, [LATP, c_int32, P, P, P, P, P, P, P, P, P, P]),
+    "funlbm_case_bytes": (c_int32, []),
+    "funlbm_case_pack": (c_int32, [LATP, c_int64, c_int64, c_int32, P]),
+    "funlbm_batch_step": (c_int32, [c_int32] + [P] * 30 + [c_int32] * 4 + [P]),
 }
 
 _lib = None

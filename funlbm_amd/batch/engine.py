@@ -1,0 +1,378 @@
+"""Many independent settling cases stepped together on one GPU.
+
+Each case is a lattice of its own size, lattice type and relaxation time with
+its own rigid particles; it follows exactly the single-lattice step of LBMD3
+(the same kernels' bodies), so a case gives bit-identical results alone and
+in a mix.  Cases share pools allocated once; the host admits a case by
+allocating ranges, packing its descriptor and initialising its state, and
+retires it by freeing the ranges.  One funlbm_batch_step launch sequence
+advances every admitted case that still has steps left; a captured HIP graph
+of several steps stays valid across admissions because all the kernels read
+their work lists and counts from device memory.
+"""
+
+import ctypes
+import dataclasses
+import math
+from typing import List, Optional
+
+import numpy as np
+import torch
+
+from .. import _native as nat
+from ..particle.markers import ellipsoid_markers
+from .alloc import RangeAllocator
+
+TILE = 256
+H_TILES, H_MARKERS, H_PARTICLES, H_CASES, H_STAMP = 0, 1, 2, 3, 4
+
+
+@dataclasses.dataclass
+class ParticleSpec:
+    """A rigid sphere or ellipsoid (the reference's particle config:
+    src/funlbm/particle/base/base.py:11-19, sphere.py, ellipsoid.py)."""
+
+    kind: str = "sphere"  # "sphere" or "ellipsoid"
+    r: float = 5.0
+    abc: tuple = (5.0, 5.0, 5.0)
+    rou: float = 1.5
+    center: tuple = (0.0, 0.0, 0.0)
+    angle: tuple = (math.pi / 2, 0.0, 0.0)  # rotation vector, alpha = pi/2 by default
+
+
+@dataclasses.dataclass
+class CaseSpec:
+    """One independent solve: a box of nodes with particles, stepped `steps` times."""
+
+    nodes: tuple
+    q: int = 19
+    tau: float = 0.6
+    steps: int = 1000
+    particles: List[ParticleSpec] = dataclasses.field(default_factory=list)
+    gravity: tuple = (0.0, 0.0, -2e-5)
+    bcs: tuple = (nat.BC_WALL,) * 6
+    uzero_mask: int = 0x3F
+
+    @property
+    def n(self):
+        return int(np.prod(self.nodes))
+
+
+_marker_cache = {}
+
+
+def markers_for(p: ParticleSpec):
+    """Body-frame markers (float32) and the constants of a particle,
+    following Sphere/Ellipsoid._init (src/funlbm/particle/base/sphere.py:15-53,
+    ellipsoid.py:107-143)."""
+    if p.kind == "sphere":
+        a = b = c = float(p.r)
+    else:
+        a, b, c = (float(v) for v in p.abc)
+    key = (p.kind, a, b, c)
+    if key not in _marker_cache:
+        lag = ellipsoid_markers(a, b, c).astype(np.float32)
+        mass = float(np.float32(4.0 / 3.0 * math.pi * a * b * c))
+        if p.kind == "sphere":
+            area = float(np.float32(4.0 * math.pi * a * a))
+            inertia = np.array([a * a] * 3) * np.float32(mass) / 5.0
+        else:
+            area = float(np.float32(4.0 / 3.0 * math.pi * math.pow(a * b * c, 2.0 / 3.0)))
+            inertia = np.array([b * c, a * c, a * b]) * np.float32(mass) / 5.0
+        lm = np.float32(np.float32(area) / np.float32(len(lag)))
+        _marker_cache[key] = (lag, mass, inertia.astype(np.float32).astype(np.float64), lm)
+    return _marker_cache[key]
+
+
+def lattice_for(spec: CaseSpec, storage: int, dev_scale: float) -> nat.Lattice:
+    lat = nat.Lattice()
+    lat.q = spec.q
+    lat.nx, lat.ny, lat.nz = (int(v) for v in spec.nodes)
+    lat.storage = storage
+    for i in range(6):
+        lat.bc[i] = int(spec.bcs[i])
+    lat.uzero_mask = spec.uzero_mask
+    lat.tau = float(np.float32(spec.tau))
+    lat.rho0 = 1.0
+    lat.dev_scale = dev_scale
+    return lat
+
+
+@dataclasses.dataclass
+class _Slot:
+    spec: CaseSpec
+    lat: nat.Lattice
+    f_off: int
+    node_off: int
+    tile_off: int
+    n_tiles: int
+    m_off: int
+    m_count: int
+    p_off: int
+    p_count: int
+
+
+class BatchEngine:
+    """Pools for up to `max_cases` cases holding at most `max_nodes` nodes,
+    `max_markers` markers and `max_particles` particles in total."""
+
+    def __init__(self, storage="f16", max_nodes=1 << 24, max_markers=1 << 16, max_particles=256,
+                 max_cases=256, max_q=27, traj_cap=64, dev_scale=1024.0, device=None):
+        from ..base.work import device_detect
+
+        self.device = device_detect(device)
+        dev = self.device
+        self.storage = nat.STORAGE_CODES[storage]
+        self.storage_name = storage
+        self.dev_scale = float(dev_scale)
+        self.max_cases = int(max_cases)
+        self.traj_cap = int(traj_cap)
+        elem_cap = int(max_nodes) * int(max_q)
+        tile_cap = int(max_nodes) // TILE + int(max_cases)
+        self.elems = RangeAllocator(elem_cap, align=64)
+        self.nodes = RangeAllocator(int(max_nodes), align=TILE)
+        self.tiles = RangeAllocator(tile_cap)
+        self.markers = RangeAllocator(int(max_markers))
+        self.parts = RangeAllocator(int(max_particles))
+        self.free_cases = list(range(self.max_cases - 1, -1, -1))
+        dt = nat.STORAGE_DTYPES[self.storage]
+        i32, f32 = torch.int32, torch.float32
+        z = lambda n, d: torch.zeros(n, device=dev, dtype=d)  # noqa: E731
+        self.pool = [z(elem_cap, dt), z(elem_cap, dt)]
+        self.fol = [z(3 * int(max_nodes), f32), z(3 * int(max_nodes), f32)]
+        self.owner = z(int(max_nodes), torch.int64)
+        self.fol_tiles = z(tile_cap, i32)
+        self.origin = [z(3 * int(max_markers), i32), z(3 * int(max_markers), i32)]
+        self.lag = z(3 * int(max_markers), f32)
+        self.lx = z(3 * int(max_markers), f32)
+        self.lu = z(3 * int(max_markers), f32)
+        self.lF = z(3 * int(max_markers), f32)
+        self.lm = z(int(max_markers), f32)
+        self.marker_case = torch.full((int(max_markers),), -1, device=dev, dtype=i32)
+        self.marker_particle = z(int(max_markers), i32)
+        self.particle_case = torch.full((int(max_particles),), -1, device=dev, dtype=i32)
+        self.particle_markers = z(2 * int(max_particles), i32)
+        self.pstate = z(int(max_particles) * nat.PSTRIDE, torch.float64)
+        self.case_bytes = nat.lib().funlbm_case_bytes()
+        self.cases = z(self.max_cases * self.case_bytes, torch.uint8)
+        self.case_par = z(self.max_cases, i32)
+        self.case_left = z(self.max_cases, i32)
+        self.case_done = z(self.max_cases, i32)
+        self.case_flag = z(self.max_cases, i32)
+        self.origin_ok = z(self.max_cases, i32)
+        self.case_marker_begin = z(self.max_cases, i32)
+        self.tile_case = torch.full((tile_cap,), -1, device=dev, dtype=i32)
+        self.tile_local = z(tile_cap, i32)
+        self.hdr = z(8, i32)
+        self.traj = z(int(max_particles) * self.traj_cap * 4, f32)
+        self.traj_cap_t = torch.tensor([self.traj_cap], device=dev, dtype=i32)
+        self.slots = {}
+        self.grid = {"tiles": 2048, "markers": 1024, "particles": 256}
+        self._graph = None
+        self._graph_steps = 0
+        self._dirty = True
+
+    # ------------------------------------------------------------------ admission
+    def fits(self, spec: CaseSpec) -> bool:
+        m = sum(len(markers_for(p)[0]) for p in spec.particles)
+        return (
+            bool(self.free_cases)
+            and self.elems.fits(spec.q * spec.n)
+            and self.nodes.fits(spec.n)
+            and self.tiles.fits(-(-spec.n // TILE))
+            and (m == 0 or self.markers.fits(m))
+            and (not spec.particles or self.parts.fits(len(spec.particles)))
+        )
+
+    @staticmethod
+    def case_bytes_needed(spec: CaseSpec, storage_bytes: int = 2) -> int:
+        """HBM held by a case: two state blocks, two FOL blocks, owner words."""
+        return spec.n * (2 * spec.q * storage_bytes + 2 * 12 + 8)
+
+    def add(self, spec: CaseSpec) -> int:
+        """Admits a case; returns its slot id.  Raises MemoryError if it does not fit."""
+        if not self.fits(spec):
+            raise MemoryError("case does not fit the pools")
+        dev = self.device
+        n = spec.n
+        f_off = self.elems.alloc(spec.q * n)
+        node_off = self.nodes.alloc(n)
+        n_tiles = -(-n // TILE)
+        tile_off = self.tiles.alloc(n_tiles)
+        infos = [markers_for(p) for p in spec.particles]
+        m_count = sum(len(i[0]) for i in infos)
+        m_off = self.markers.alloc(m_count) if m_count else 0
+        p_off = self.parts.alloc(len(spec.particles)) if spec.particles else 0
+        cid = self.free_cases.pop()
+        lat = lattice_for(spec, self.storage, self.dev_scale)
+        desc = (ctypes.c_uint8 * self.case_bytes)()
+        nat.call("funlbm_case_pack", lat, f_off, node_off, tile_off, desc)
+        self.cases[cid * self.case_bytes : (cid + 1) * self.case_bytes] = torch.frombuffer(
+            bytearray(desc), dtype=torch.uint8
+        ).to(dev)
+        # state: the equilibrium at rho = 1, u = 0 (the reference's init and
+        # its two collisions before the first stream leave f = w)
+        esz = self.pool[0].element_size()
+        nat.call("funlbm_equilibrium_state", lat, None, None,
+                 ctypes.c_void_p(self.pool[0].data_ptr() + f_off * esz), nat.stream_handle(dev))
+        for j in range(2):
+            self.fol[j][3 * node_off : 3 * (node_off + n)].zero_()
+        self.fol_tiles[tile_off : tile_off + n_tiles].zero_()
+        # markers and particles
+        ps = np.zeros((len(spec.particles), nat.PSTRIDE))
+        pm = np.zeros(2 * len(spec.particles), dtype=np.int32)
+        lags, lms, mp = [], [], []
+        cur = m_off
+        for k, (p, (lag, mass, inertia, lm)) in enumerate(zip(spec.particles, infos)):
+            lags.append(lag)
+            lms.append(np.full(len(lag), lm, np.float32))
+            mp.append(np.full(len(lag), p_off + k, np.int32))
+            pm[2 * k], pm[2 * k + 1] = cur, cur + len(lag)
+            cur += len(lag)
+            ps[k, nat.P_CX : nat.P_CX + 3] = np.array(p.center, np.float32)
+            ps[k, nat.P_ANGLE : nat.P_ANGLE + 3] = np.array(p.angle, np.float32)
+            ps[k, nat.P_MASS] = mass
+            ps[k, nat.P_INERTIA : nat.P_INERTIA + 3] = inertia
+            ps[k, nat.P_RHO] = p.rou
+            ps[k, nat.P_GRAV : nat.P_GRAV + 3] = np.array(spec.gravity, np.float32)
+        if m_count:
+            sl = slice(3 * m_off, 3 * (m_off + m_count))
+            self.lag[sl] = torch.from_numpy(np.concatenate(lags).reshape(-1)).to(dev)
+            self.lx[sl].zero_()
+            self.lu[sl].zero_()
+            self.lF[sl].zero_()
+            self.lm[m_off : m_off + m_count] = torch.from_numpy(np.concatenate(lms)).to(dev)
+            self.marker_particle[m_off : m_off + m_count] = torch.from_numpy(np.concatenate(mp)).to(dev)
+            self.marker_case[m_off : m_off + m_count] = cid
+        if spec.particles:
+            P = len(spec.particles)
+            self.pstate[p_off * nat.PSTRIDE : (p_off + P) * nat.PSTRIDE] = torch.from_numpy(
+                ps.reshape(-1)).to(dev)
+            self.particle_markers[2 * p_off : 2 * (p_off + P)] = torch.from_numpy(pm).to(dev)
+            self.particle_case[p_off : p_off + P] = cid
+            self.traj[p_off * self.traj_cap * 4 : (p_off + P) * self.traj_cap * 4].zero_()
+        dyn = torch.tensor([0, int(spec.steps), 0, 0, 0, m_off], dtype=torch.int32).to(dev)
+        self.case_par[cid] = dyn[0]
+        self.case_left[cid] = dyn[1]
+        self.case_done[cid] = dyn[2]
+        self.case_flag[cid] = dyn[3]
+        self.origin_ok[cid] = dyn[4]
+        self.case_marker_begin[cid] = dyn[5]
+        self.slots[cid] = _Slot(spec, lat, f_off, node_off, tile_off, n_tiles, m_off, m_count, p_off,
+                                len(spec.particles))
+        self._dirty = True
+        return cid
+
+    def remove(self, cid: int):
+        s = self.slots.pop(cid)
+        self.case_left[cid] = 0
+        if s.m_count:
+            self.marker_case[s.m_off : s.m_off + s.m_count] = -1
+            self.markers.free(s.m_off, s.m_count)
+        if s.p_count:
+            self.particle_case[s.p_off : s.p_off + s.p_count] = -1
+            self.parts.free(s.p_off, s.p_count)
+        self.elems.free(s.f_off, s.spec.q * s.spec.n)
+        self.nodes.free(s.node_off, s.spec.n)
+        self.tiles.free(s.tile_off, s.n_tiles)
+        self.free_cases.append(cid)
+        self._dirty = True
+
+    def _publish(self):
+        """Rebuilds the active tile list and the header counts (host -> device)."""
+        if not self._dirty:
+            return
+        tc, tl = [], []
+        for cid, s in sorted(self.slots.items()):
+            tc.append(np.full(s.n_tiles, cid, np.int32))
+            tl.append(np.arange(s.n_tiles, dtype=np.int32))
+        nt = int(sum(len(a) for a in tc))
+        if nt:
+            self.tile_case[:nt] = torch.from_numpy(np.concatenate(tc)).to(self.device)
+            self.tile_local[:nt] = torch.from_numpy(np.concatenate(tl)).to(self.device)
+        stamp = int(self.hdr[H_STAMP].item())
+        hdr = [nt, self.markers.high_water, self.parts.high_water, self.max_cases, stamp, 0, 0, 0]
+        self.hdr.copy_(torch.tensor(hdr, dtype=torch.int32))
+        self._with_ibm = 1 if self.markers.high_water > 0 else 0
+        self._dirty = False
+
+    # ------------------------------------------------------------------ stepping
+    def _launch(self):
+        P = nat.ptr
+        g = self.grid
+        nat.call(
+            "funlbm_batch_step", self.storage, P(self.cases), P(self.hdr), P(self.case_par),
+            P(self.case_left), P(self.case_done), P(self.case_flag), P(self.tile_case),
+            P(self.tile_local), P(self.marker_case), P(self.marker_particle), P(self.case_marker_begin),
+            P(self.particle_case), P(self.particle_markers), P(self.pool[0]), P(self.pool[1]),
+            P(self.fol[0]), P(self.fol[1]), P(self.owner), P(self.fol_tiles), P(self.origin[0]),
+            P(self.origin[1]), P(self.origin_ok), P(self.lag), P(self.lx), P(self.lu), P(self.lF),
+            P(self.lm), P(self.pstate), P(self.traj), P(self.traj_cap_t), int(g["tiles"]),
+            int(g["markers"]), int(g["particles"]), 1, nat.stream_handle(self.device),
+        )
+
+    def step(self, n: int = 1):
+        """n timesteps of every case with steps left (eager launches)."""
+        self._publish()
+        for _ in range(n):
+            self._launch()
+
+    def capture(self, n: int):
+        """Captures n steps in a HIP graph (replayed by run_graph)."""
+        self._publish()
+        # warm the launch path outside the capture
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream(device=self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                for _ in range(n):
+                    self._launch()
+        torch.cuda.current_stream(self.device).wait_stream(s)
+        self._graph, self._graph_steps = g, n
+        return g
+
+    def run_graph(self):
+        self._publish()
+        self._graph.replay()
+
+    # ------------------------------------------------------------------ results
+    def left(self):
+        return self.case_left.cpu().numpy()
+
+    def finished(self):
+        left = self.left()
+        return [cid for cid in self.slots if left[cid] <= 0]
+
+    def particle_rows(self, cid):
+        s = self.slots[cid]
+        return self.pstate[s.p_off * nat.PSTRIDE : (s.p_off + s.p_count) * nat.PSTRIDE].view(
+            s.p_count, nat.PSTRIDE)
+
+    def trajectory(self, cid):
+        """Recorded (cx, cy, cz, step) per particle, ring of traj_cap entries."""
+        s = self.slots[cid]
+        t = self.traj.view(-1, self.traj_cap, 4)[s.p_off : s.p_off + s.p_count]
+        return t
+
+    def fields(self, cid):
+        """The reference's u [X,Y,Z,3] and rho [X,Y,Z,1] of a case after its last step."""
+        s = self.slots[cid]
+        dev = self.device
+        n = s.spec.n
+        par = int(self.case_par[cid].item())
+        a = 1 - par  # state streamed by the last step
+        esz = self.pool[0].element_size()
+        u = torch.empty((3, n), device=dev, dtype=torch.float32)
+        rho = torch.empty((n,), device=dev, dtype=torch.float32)
+        fol_new = self.fol[par][3 * s.node_off :]
+        fol_old = self.fol[a][3 * s.node_off :]
+        stamp = self.hdr[H_STAMP : H_STAMP + 1]
+        nat.call(
+            "funlbm_materialize", s.lat, ctypes.c_void_p(self.pool[a].data_ptr() + s.f_off * esz),
+            nat.ptr(fol_old), nat.ptr(fol_new), nat.ptr(self.fol_tiles[s.tile_off :]), nat.ptr(stamp),
+            None, None, nat.ptr(u), nat.ptr(rho), None, nat.stream_handle(dev),
+        )
+        X, Y, Z = s.spec.nodes
+        return u.view(3, X, Y, Z).permute(1, 2, 3, 0), rho.view(X, Y, Z, 1)

@@ -1,0 +1,339 @@
+// Many independent lattices of uneven size stepped together.
+//
+// Every case owns contiguous blocks of shared pools: [q][n] populations in
+// each of the two state pools, [3][n] forces in each of the two FOL pools,
+// n owner words, and ceil(n/256) entries of the tile table.  A case
+// descriptor (CaseDesc, built on the host by funlbm_case_pack) carries its
+// lattice parameters and offsets.  Per-case device counters hold the step
+// parity (which pool is the source), the steps left and the steps done; a
+// case whose steps-left reach zero stops moving while the others go on.
+//
+// The kernels are persistent grid-stride loops over device-side work lists
+// (active tiles, markers, particles, cases) whose lengths live in a device
+// header, so one captured HIP graph of a step sequence stays valid while the
+// host admits and retires cases between replays.  Per step:
+//   interp -> wall -> forces -> spread_prep -> spread_write -> fused -> pose -> advance
+#include "capi_util.h"
+#include "fused_node.h"
+#include "ibm_body.h"
+#include "lbm_common.h"
+
+namespace funlbm {
+
+// Device header (int32): counts of the work lists and the spreading stamp.
+enum : int {
+  H_TILES = 0,      // active tiles in tile_case / tile_local
+  H_MARKERS = 1,    // markers in the marker arrays
+  H_PARTICLES = 2,  // particles in the particle arrays
+  H_CASES = 3,      // cases in the case arrays
+  H_STAMP = 4,      // stamp of the last completed spreading
+  H_WORDS = 8
+};
+
+struct CaseDesc {
+  KParams p;
+  int32_t q;
+  int32_t tile_begin;  // first entry of the case in the tile table
+  int32_t n_tiles;
+  int32_t pad0;
+  int64_t f_off;     // element offset of the [q][n] block in each state pool
+  int64_t node_off;  // node offset: FOL blocks at 3*node_off, owner at node_off
+};
+
+struct BatchDyn {
+  int32_t* par;   // [C] 0: state in pool A / FOL_old in FOL A; 1: the other way
+  int32_t* left;  // [C] steps still to take
+  int32_t* done;  // [C] steps taken
+  int32_t* flag;  // [C] weight-sum failure flag
+};
+
+template <class Fn>
+__device__ inline void with_lattice(int q, Fn&& fn) {
+  switch (q) {
+    case 13: fn(D3Q13{}); break;
+    case 15: fn(D3Q15{}); break;
+    case 19: fn(D3Q19{}); break;
+    default: fn(D3Q27{}); break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Fused step over the active tiles (256 nodes of one case each).
+// ---------------------------------------------------------------------------
+template <class S>
+__global__ __launch_bounds__(kBlock) void k_batch_fused(
+    const CaseDesc* __restrict__ cases, const int* __restrict__ tile_case,
+    const int* __restrict__ tile_local, const int* __restrict__ hdr, BatchDyn dyn,
+    typename S::elem* pool_a, typename S::elem* pool_b, const float* fol_a, const float* fol_b,
+    const uint32_t* __restrict__ fol_tiles) {
+  using E = typename S::elem;
+  const int n_tiles = hdr[H_TILES];
+  const uint32_t stamp = (uint32_t)hdr[H_STAMP] + 1u;  // this step's spreading
+  for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const int c = tile_case[t];
+    if (c < 0 || dyn.left[c] <= 0) continue;
+    const CaseDesc& cd = cases[c];
+    const int lt = tile_local[t];
+    const int64_t node = (int64_t)lt * kBlock + threadIdx.x;
+    if (node >= cd.p.n) continue;
+    const int par = dyn.par[c];
+    E* pa = pool_a + cd.f_off;
+    E* pb = pool_b + cd.f_off;
+    const E* src = par ? pb : pa;
+    E* dst = par ? pa : pb;
+    const float* fa = fol_a + 3 * cd.node_off;
+    const float* fb = fol_b + 3 * cd.node_off;
+    const float* fo = par ? fb : fa;
+    const float* fn = par ? fa : fb;
+    const uint32_t entry = fol_tiles[cd.tile_begin + lt];
+    const bool new_here = entry == stamp;
+    const bool old_here = entry == stamp || entry == stamp - 1u;
+    FusedIO<S> io{src, dst, fo, fn, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    with_lattice(cd.q, [&](auto l) {
+      using L = decltype(l);
+      fused_node<L, S, true, false>(cd.p, node, io, new_here, old_here);
+    });
+  }
+}
+
+// ---------------------------------------------------------------------------
+// IBM over all markers / particles of stepping cases.
+// ---------------------------------------------------------------------------
+template <class S>
+__global__ __launch_bounds__(kIbmBlock) void k_batch_interp(
+    const CaseDesc* __restrict__ cases, const int* __restrict__ hdr, BatchDyn dyn,
+    const int* __restrict__ marker_case, const int* __restrict__ marker_particle,
+    const typename S::elem* __restrict__ pool_a, const typename S::elem* __restrict__ pool_b,
+    const float* __restrict__ fol_a, const float* __restrict__ fol_b,
+    const float* __restrict__ lx, const double* __restrict__ ps, float* __restrict__ lu,
+    float* __restrict__ lF) {
+  const int nm = hdr[H_MARKERS];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int waves = gridDim.x * (kIbmBlock / kWave);
+  for (int m = blockIdx.x * (kIbmBlock / kWave) + threadIdx.x / kWave; m < nm; m += waves) {
+    const int c = marker_case[m];
+    if (c < 0 || dyn.left[c] <= 0) continue;
+    const CaseDesc& cd = cases[c];
+    const int par = dyn.par[c];
+    const typename S::elem* st = (par ? pool_b : pool_a) + cd.f_off;
+    const float* fo = (par ? fol_b : fol_a) + 3 * cd.node_off;
+    const bool check = dyn.done[c] + 1 > 100;
+    with_lattice(cd.q, [&](auto l) {
+      using L = decltype(l);
+      interp_marker<L, S, true>(cd.p, m, lane, st, fo, nullptr, nullptr, nullptr, lx,
+                                pst(ps, marker_particle[m]), 1.0f, check, lu, lF, &dyn.flag[c]);
+    });
+  }
+}
+
+// particle_markers holds [begin, end) marker ranges, two int32 per particle.
+__global__ __launch_bounds__(kIbmBlock) void k_batch_wall(const CaseDesc* __restrict__ cases,
+                                                          const int* __restrict__ hdr, BatchDyn dyn,
+                                                          const int* __restrict__ particle_case,
+                                                          const int* __restrict__ pm,
+                                                          const float* __restrict__ lx,
+                                                          float* __restrict__ lF, float dx) {
+  const int np = hdr[H_PARTICLES];
+  for (int i = blockIdx.x; i < np; i += gridDim.x) {
+    const int c = particle_case[i];
+    if (c < 0 || dyn.left[c] <= 0) continue;
+    wall_particle(cases[c].p, pm[2 * i], pm[2 * i + 1], lx, lF, dx);
+  }
+}
+
+__global__ __launch_bounds__(kIbmBlock) void k_batch_forces(const int* __restrict__ hdr, BatchDyn dyn,
+                                                            const int* __restrict__ particle_case,
+                                                            const int* __restrict__ pm,
+                                                            const float* __restrict__ lx,
+                                                            const float* __restrict__ lF,
+                                                            const float* __restrict__ lm,
+                                                            double* __restrict__ ps) {
+  const int np = hdr[H_PARTICLES];
+  for (int i = blockIdx.x; i < np; i += gridDim.x) {
+    const int c = particle_case[i];
+    if (c < 0 || dyn.left[c] <= 0) continue;
+    forces_particle(pm[2 * i], pm[2 * i + 1], lx, lF, lm, pst(ps, i), 1.0, 1.0);
+  }
+}
+
+__global__ __launch_bounds__(kIbmBlock) void k_batch_spread_prep(
+    const CaseDesc* __restrict__ cases, const int* __restrict__ hdr, BatchDyn dyn,
+    const int* __restrict__ marker_case, const int* __restrict__ case_marker_begin,
+    const float* __restrict__ lx, const int* __restrict__ origin_a, const int* __restrict__ origin_b,
+    const int* __restrict__ origin_ok, float* __restrict__ fol_a, float* __restrict__ fol_b,
+    unsigned long long* __restrict__ owner) {
+  const int nm = hdr[H_MARKERS];
+  const uint32_t stamp = (uint32_t)hdr[H_STAMP] + 1u;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int waves = gridDim.x * (kIbmBlock / kWave);
+  for (int m = blockIdx.x * (kIbmBlock / kWave) + threadIdx.x / kWave; m < nm; m += waves) {
+    const int c = marker_case[m];
+    if (c < 0 || dyn.left[c] <= 0) continue;
+    const CaseDesc& cd = cases[c];
+    const int par = dyn.par[c];
+    // FOL_new of this step is buffer 1 - par; its origins were recorded the
+    // last time the case wrote that buffer
+    float* fn = (par ? fol_a : fol_b) + 3 * cd.node_off;
+    const int* org = par ? origin_a : origin_b;
+    const bool ok = (origin_ok[c] >> (1 - par)) & 1;
+    spread_prep_marker(cd.p, m, m - case_marker_begin[c], lane, lx, ok ? org : nullptr, fn,
+                       owner + cd.node_off, stamp);
+  }
+}
+
+__global__ __launch_bounds__(kIbmBlock) void k_batch_spread_write(
+    const CaseDesc* __restrict__ cases, const int* __restrict__ hdr, BatchDyn dyn,
+    const int* __restrict__ marker_case, const int* __restrict__ case_marker_begin,
+    const float* __restrict__ lx, const float* __restrict__ lF, const float* __restrict__ lm,
+    int* __restrict__ origin_a, int* __restrict__ origin_b, int* __restrict__ origin_ok,
+    float* __restrict__ fol_a, float* __restrict__ fol_b,
+    const unsigned long long* __restrict__ owner, uint32_t* __restrict__ fol_tiles) {
+  const int nm = hdr[H_MARKERS];
+  const uint32_t stamp = (uint32_t)hdr[H_STAMP] + 1u;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int waves = gridDim.x * (kIbmBlock / kWave);
+  for (int m = blockIdx.x * (kIbmBlock / kWave) + threadIdx.x / kWave; m < nm; m += waves) {
+    const int c = marker_case[m];
+    if (c < 0 || dyn.left[c] <= 0) continue;
+    const CaseDesc& cd = cases[c];
+    const int par = dyn.par[c];
+    float* fn = (par ? fol_a : fol_b) + 3 * cd.node_off;
+    int* org = par ? origin_a : origin_b;
+    spread_write_marker(cd.p, m, m - case_marker_begin[c], lane, lx, lF, lm, fn, owner + cd.node_off,
+                        org, fol_tiles + cd.tile_begin, stamp);
+    if (lane == 0 && m == case_marker_begin[c]) atomicOr(&origin_ok[c], 1 << (1 - par));
+  }
+}
+
+__global__ __launch_bounds__(kIbmBlock) void k_batch_pose(const int* __restrict__ hdr, BatchDyn dyn,
+                                                          const int* __restrict__ particle_case,
+                                                          const int* __restrict__ pm,
+                                                          const float* __restrict__ lag,
+                                                          float* __restrict__ lx,
+                                                          double* __restrict__ ps,
+                                                          float* __restrict__ traj,
+                                                          const int* __restrict__ traj_cap) {
+  const int np = hdr[H_PARTICLES];
+  for (int i = blockIdx.x; i < np; i += gridDim.x) {
+    const int c = particle_case[i];
+    if (c < 0 || dyn.left[c] <= 0) continue;
+    double* P = pst(ps, i);
+    pose_particle(pm[2 * i], pm[2 * i + 1], lag, lx, P, true);
+    if (traj && threadIdx.x == 0) {
+      // centroid after this step, ring of traj_cap[0] entries per particle
+      const int cap = traj_cap[0];
+      const int slot = dyn.done[c] % cap;
+      float* t = traj + ((int64_t)i * cap + slot) * 4;
+      t[0] = (float)P[FUNLBM_P_CX];
+      t[1] = (float)P[FUNLBM_P_CX + 1];
+      t[2] = (float)P[FUNLBM_P_CX + 2];
+      t[3] = (float)(dyn.done[c] + 1);
+    }
+  }
+}
+
+// Case bookkeeping after a step: parity flips, counters move, the stamp of
+// the spreading just completed becomes current.
+__global__ void k_batch_advance(int* __restrict__ hdr, BatchDyn dyn) {
+  const int nc = hdr[H_CASES];
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+    if (dyn.left[c] > 0) {
+      dyn.par[c] ^= 1;
+      dyn.left[c] -= 1;
+      dyn.done[c] += 1;
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) hdr[H_STAMP] += 1;
+}
+
+}  // namespace funlbm
+
+using namespace funlbm;
+
+extern "C" {
+
+int funlbm_case_bytes(void) { return (int)sizeof(CaseDesc); }
+
+int funlbm_case_pack(const funlbm_lattice* lat, int64_t f_offset, int64_t node_offset,
+                     int32_t tile_begin, void* desc_out) {
+  if (!desc_out) return set_error(FUNLBM_EINVAL, "case_pack: NULL output");
+  CaseDesc d;
+  memset(&d, 0, sizeof(d));
+  int rc = make_params(lat, &d.p);
+  if (rc) return rc;
+  if (d.p.has_moving)
+    return set_error(FUNLBM_EUNSUPPORTED_BC, "batched cases take PERIODICAL and WALL faces only");
+  if (f_offset < 0 || node_offset < 0 || tile_begin < 0)
+    return set_error(FUNLBM_EINVAL, "case_pack: negative offset");
+  d.q = lat->q;
+  d.tile_begin = tile_begin;
+  d.n_tiles = (int32_t)((d.p.n + kBlock - 1) / kBlock);
+  d.f_off = f_offset;
+  d.node_off = node_offset;
+  memcpy(desc_out, &d, sizeof(d));
+  return 0;
+}
+
+int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, int32_t* case_par,
+                      int32_t* case_left, int32_t* case_done, int32_t* case_flag,
+                      const int32_t* tile_case, const int32_t* tile_local,
+                      const int32_t* marker_case, const int32_t* marker_particle,
+                      const int32_t* case_marker_begin, const int32_t* particle_case,
+                      const int32_t* particle_markers, void* pool_a, void* pool_b, float* fol_a,
+                      float* fol_b, uint64_t* owner, uint32_t* fol_tiles, int32_t* origin_a,
+                      int32_t* origin_b, int32_t* origin_ok, const float* lag, float* lx, float* lu,
+                      float* lF, const float* lm, double* pstate, float* traj,
+                      const int32_t* traj_cap, int32_t grid_tiles, int32_t grid_markers,
+                      int32_t grid_particles, int32_t with_ibm, void* stream) {
+  if (!cases || !hdr || !case_par || !case_left || !case_done || !case_flag || !tile_case ||
+      !tile_local || !pool_a || !pool_b || !fol_a || !fol_b || !fol_tiles)
+    return set_error(FUNLBM_EINVAL, "batch_step: NULL array");
+  if (grid_tiles <= 0 || grid_markers <= 0 || grid_particles <= 0)
+    return set_error(FUNLBM_EINVAL, "batch_step: grid sizes must be positive");
+  if (with_ibm && (!marker_case || !marker_particle || !case_marker_begin || !particle_case ||
+                   !particle_markers || !owner || !origin_a || !origin_b || !origin_ok || !lag ||
+                   !lx || !lu || !lF || !lm || !pstate))
+    return set_error(FUNLBM_EINVAL, "batch_step: NULL particle array");
+  const CaseDesc* cd = (const CaseDesc*)cases;
+  int32_t* h = (int32_t*)hdr;
+  BatchDyn dyn{case_par, case_left, case_done, case_flag};
+  hipStream_t s = (hipStream_t)stream;
+  int rc = 0;
+  return dispatch_storage(storage, [&](auto st) {
+    using S = decltype(st);
+    using E = typename S::elem;
+    if (with_ibm) {
+      hipLaunchKernelGGL((k_batch_interp<S>), dim3(grid_markers), dim3(kIbmBlock), 0, s, cd, hdr, dyn,
+                         marker_case, marker_particle, (const E*)pool_a, (const E*)pool_b, fol_a,
+                         fol_b, lx, pstate, lu, lF);
+      if ((rc = check_launch("batch_interp"))) return rc;
+      hipLaunchKernelGGL(k_batch_wall, dim3(grid_particles), dim3(kIbmBlock), 0, s, cd, hdr, dyn,
+                         particle_case, particle_markers, lx, lF, 1.0f);
+      if ((rc = check_launch("batch_wall"))) return rc;
+      hipLaunchKernelGGL(k_batch_forces, dim3(grid_particles), dim3(kIbmBlock), 0, s, hdr, dyn,
+                         particle_case, particle_markers, lx, lF, lm, pstate);
+      if ((rc = check_launch("batch_forces"))) return rc;
+      hipLaunchKernelGGL(k_batch_spread_prep, dim3(grid_markers), dim3(kIbmBlock), 0, s, cd, hdr, dyn,
+                         marker_case, case_marker_begin, lx, origin_a, origin_b, origin_ok, fol_a,
+                         fol_b, (unsigned long long*)owner);
+      if ((rc = check_launch("batch_spread_prep"))) return rc;
+      hipLaunchKernelGGL(k_batch_spread_write, dim3(grid_markers), dim3(kIbmBlock), 0, s, cd, hdr,
+                         dyn, marker_case, case_marker_begin, lx, lF, lm, origin_a, origin_b,
+                         origin_ok, fol_a, fol_b, (const unsigned long long*)owner, fol_tiles);
+      if ((rc = check_launch("batch_spread_write"))) return rc;
+    }
+    hipLaunchKernelGGL((k_batch_fused<S>), dim3(grid_tiles), dim3(kBlock), 0, s, cd, tile_case,
+                       tile_local, hdr, dyn, (E*)pool_a, (E*)pool_b, (const float*)fol_a,
+                       (const float*)fol_b, (const uint32_t*)fol_tiles);
+    if ((rc = check_launch("batch_fused"))) return rc;
+    if (with_ibm) {
+      hipLaunchKernelGGL(k_batch_pose, dim3(grid_particles), dim3(kIbmBlock), 0, s, hdr, dyn,
+                         particle_case, particle_markers, lag, lx, pstate, traj, traj_cap);
+      if ((rc = check_launch("batch_pose"))) return rc;
+    }
+    hipLaunchKernelGGL(k_batch_advance, dim3(4), dim3(256), 0, s, h, dyn);
+    return check_launch("batch_advance");
+  });
+}
+
+}  // extern "C"

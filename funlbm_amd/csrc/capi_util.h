@@ -6,6 +6,7 @@
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <string.h>
 
 #include <algorithm>
 
@@ -63,6 +64,9 @@ inline int make_params(const funlbm_lattice* lat, KParams* p) {
   p->ny = lat->ny;
   p->nz = lat->nz;
   p->n = (int64_t)lat->nx * lat->ny * lat->nz;
+  if (p->n > 0x7fffffffLL)
+    return set_error(FUNLBM_ETOOBIG, "%lld nodes exceed the 2^31 - 1 node limit", (long long)p->n);
+  p->sxy = lat->ny * lat->nz;
   p->has_moving = 0;
   for (int f = 0; f < 6; ++f) {
     const int bc = lat->bc[f];

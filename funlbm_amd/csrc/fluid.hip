@@ -4,11 +4,11 @@
 // flattened 1-D grid of 256-thread blocks, so rows of any length keep every
 // lane busy and every access to a direction plane is a coalesced row.
 #include "capi_util.h"
+#include "fused_node.h"
 #include "lbm_common.h"
 
 namespace funlbm {
 
-constexpr int kBlock = 256;
 
 // ---------------------------------------------------------------------------
 // Explicit (reference-structured) kernels, float, one pass each.
@@ -138,69 +138,26 @@ __global__ __launch_bounds__(kBlock) void k_equilibrium(KParams p, const float* 
 }
 
 // ---------------------------------------------------------------------------
-// Fused pull step.  FORCE: the lattice has a forcing field (IBM FOL and/or a
-// body force).  MAT: write the reference's f/u/rho/p instead of colliding.
+// Fused pull step, one node per thread (body in fused_node.h).
 // ---------------------------------------------------------------------------
 template <class L, class S, bool FORCE, bool MAT>
 __global__ __launch_bounds__(kBlock) void k_fused(
     KParams p, const typename S::elem* __restrict__ src, typename S::elem* __restrict__ dst,
     const float* __restrict__ fol_old, const float* __restrict__ fol_new,
+    const uint32_t* __restrict__ fol_tiles, const int* __restrict__ ctl,
     const float* __restrict__ rho_prev, float* __restrict__ rho_next, float* __restrict__ out_f,
     float* __restrict__ out_u, float* __restrict__ out_rho, float* __restrict__ out_p) {
-  using T = typename S::T;
   const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
   const int64_t node = (int64_t)b * kBlock + threadIdx.x;
   if (node >= p.n) return;
-  const Node c = node_coords(p, node);
-  T f[L::Q];
-  pull_node<L, S>(p, src, c, node, rho_prev, f);
-
-  T Fn[3] = {(T)0, (T)0, (T)0};
-  if (FORCE) {
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-      Fn[d] = (T)(fol_new ? fol_new[(int64_t)d * p.n + node] + p.g[d] : p.g[d]);
-    if (Fn[0] != (T)0 || Fn[1] != (T)0 || Fn[2] != (T)0) {
-      // u' of the first moment update (old FOL), then cul_equ2 with the new FOL.
-      T Fo[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-        Fo[d] = (T)(fol_old ? fol_old[(int64_t)d * p.n + node] + p.g[d] : p.g[d]);
-      T r1, u1[3];
-      moments<L, T>(p, c, f, Fo, r1, u1);
-      add_forcing<L, T>(f, u1, Fn, (T)p.tau);
-    }
+  bool new_here = fol_new != nullptr, old_here = fol_old != nullptr;
+  if (FORCE && fol_tiles) {
+    const uint32_t entry = fol_tiles[(uint32_t)node >> 8], stamp = (uint32_t)ctl[0];
+    new_here = new_here && entry == stamp;
+    old_here = old_here && (entry == stamp || entry == stamp - 1u);
   }
-  T rho, u[3];
-  moments<L, T>(p, c, f, Fn, rho, u);
-
-  if constexpr (MAT) {
-    if (out_f) {
-#pragma unroll
-      for (int k = 0; k < L::Q; ++k) out_f[(int64_t)k * p.n + node] = (float)f[k];
-    }
-    if (out_u) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d) out_u[(int64_t)d * p.n + node] = (float)u[d];
-    }
-    if (out_rho) out_rho[node] = (float)rho;
-    if (out_p) out_p[node] = (float)(rho / (T)3);
-    return;
-  } else {
-    collide<L, T>(f, u, rho, (T)p.tau, (T*)nullptr);
-    if (p.has_moving && rho_next) {
-      bool mf = false;
-#pragma unroll
-      for (int face = 0; face < 6; ++face)
-        mf |= (p.bc[face] == FUNLBM_BC_WALL_WITH_SPEED) && on_face(p, c, face);
-      if (mf) rho_next[node] = (float)rho;
-    }
-    static_for<0, L::Q>([&](auto kc) {
-      constexpr int K = decltype(kc)::value;
-      const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
-      dst[(int64_t)K * p.n + node] = S::store(f[K], wr, p.inv_scale, p.scale);
-    });
-  }
+  FusedIO<S> io{src, dst, fol_old, fol_new, rho_prev, rho_next, out_f, out_u, out_rho, out_p};
+  fused_node<L, S, FORCE, MAT>(p, node, io, new_here, old_here);
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy(const uint4* __restrict__ src,
@@ -332,8 +289,9 @@ int funlbm_equilibrium_state(const funlbm_lattice* lat, const float* rho, const 
 }
 
 static int fused_launch(const funlbm_lattice* lat, const void* src, void* dst, const float* fol_old,
-                        const float* fol_new, const float* rho_prev, float* rho_next, float* out_f,
-                        float* out_u, float* out_rho, float* out_p, bool mat, void* stream) {
+                        const float* fol_new, const uint32_t* fol_tiles, const int32_t* ctl,
+                        const float* rho_prev, float* rho_next, float* out_f, float* out_u,
+                        float* out_rho, float* out_p, bool mat, void* stream) {
   KParams p;
   int rc = make_params(lat, &p);
   if (rc) return rc;
@@ -341,6 +299,7 @@ static int fused_launch(const funlbm_lattice* lat, const void* src, void* dst, c
   if (!mat && src == dst) return set_error(FUNLBM_EINVAL, "fused: source and destination alias");
   if (p.has_moving && !rho_prev)
     return set_error(FUNLBM_EINVAL, "fused: moving wall needs the previous face density");
+  if (fol_tiles && !ctl) return set_error(FUNLBM_EINVAL, "fused: a FOL tile table needs ctl");
   const bool force = fol_old || fol_new || p.has_body;
   hipStream_t s = (hipStream_t)stream;
   return dispatch_lattice(lat->q, [&](auto l) {
@@ -350,7 +309,8 @@ static int fused_launch(const funlbm_lattice* lat, const void* src, void* dst, c
       using E = typename S::elem;
       auto go = [&](auto kern) {
         hipLaunchKernelGGL(kern, dim3(grid_for(p.n)), dim3(kBlock), 0, s, p, (const E*)src, (E*)dst,
-                           fol_old, fol_new, rho_prev, rho_next, out_f, out_u, out_rho, out_p);
+                           fol_old, fol_new, fol_tiles, ctl, rho_prev, rho_next, out_f, out_u, out_rho,
+                           out_p);
         return check_launch(mat ? "materialize" : "fused_step");
       };
       if (mat) return force ? go(k_fused<L, S, true, true>) : go(k_fused<L, S, false, true>);
@@ -360,17 +320,18 @@ static int fused_launch(const funlbm_lattice* lat, const void* src, void* dst, c
 }
 
 int funlbm_fused_step(const funlbm_lattice* lat, const void* state_src, void* state_dst,
-                      const float* fol_old, const float* fol_new, const float* rho_prev,
-                      float* rho_next, void* stream) {
-  return fused_launch(lat, state_src, state_dst, fol_old, fol_new, rho_prev, rho_next, nullptr,
-                      nullptr, nullptr, nullptr, false, stream);
+                      const float* fol_old, const float* fol_new, const uint32_t* fol_tiles,
+                      const int32_t* ctl, const float* rho_prev, float* rho_next, void* stream) {
+  return fused_launch(lat, state_src, state_dst, fol_old, fol_new, fol_tiles, ctl, rho_prev, rho_next,
+                      nullptr, nullptr, nullptr, nullptr, false, stream);
 }
 
 int funlbm_materialize(const funlbm_lattice* lat, const void* state_src, const float* fol_old,
-                       const float* fol_new, const float* rho_prev, float* f, float* u, float* rho,
-                       float* pr, void* stream) {
-  return fused_launch(lat, state_src, nullptr, fol_old, fol_new, rho_prev, nullptr, f, u, rho, pr,
-                      true, stream);
+                       const float* fol_new, const uint32_t* fol_tiles, const int32_t* ctl,
+                       const float* rho_prev, float* f, float* u, float* rho, float* pr,
+                       void* stream) {
+  return fused_launch(lat, state_src, nullptr, fol_old, fol_new, fol_tiles, ctl, rho_prev, nullptr, f,
+                      u, rho, pr, true, stream);
 }
 
 int funlbm_copy_calibrate(const void* src, void* dst, int64_t n_bytes, void* stream) {

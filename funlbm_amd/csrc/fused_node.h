@@ -1,0 +1,87 @@
+// Per-node body of the fused timestep, shared by the single-lattice kernel
+// (fluid.hip) and the multi-lattice kernel (batch.hip).
+//
+// For node `node` of lattice p: pull-stream the post-collision state with the
+// face overrides; where FOL_new != 0 add the forcing term with u' from
+// FOL_old; moments with FOL_new; zero u on the masked faces; then either
+// collide and store (MAT = false) or write the reference's f, u, rho, p
+// (MAT = true).  Follows src/funlbm/lbm/base.py:159-175 and the cul_equ that
+// opens the next step (:161).
+#pragma once
+
+#include "lbm_common.h"
+
+namespace funlbm {
+
+constexpr int kBlock = 256;  // nodes per block; also the tile of the FOL tile table
+
+template <class S>
+struct FusedIO {
+  const typename S::elem* __restrict__ src;
+  typename S::elem* __restrict__ dst;
+  const float* __restrict__ fol_old;
+  const float* __restrict__ fol_new;
+  const float* __restrict__ rho_prev;
+  float* __restrict__ rho_next;
+  float* __restrict__ out_f;
+  float* __restrict__ out_u;
+  float* __restrict__ out_rho;
+  float* __restrict__ out_p;
+};
+
+template <class L, class S, bool FORCE, bool MAT>
+__device__ __forceinline__ void fused_node(const KParams& p, int64_t node, const FusedIO<S>& io,
+                                           bool new_here, bool old_here) {
+  using T = typename S::T;
+  const Node c = node_coords(p, node);
+  T f[L::Q];
+  pull_node<L, S>(p, io.src, c, node, io.rho_prev, f);
+
+  T Fn[3] = {(T)0, (T)0, (T)0};
+  if (FORCE) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      Fn[d] = (T)(new_here ? io.fol_new[(int64_t)d * p.n + node] + p.g[d] : p.g[d]);
+    if (Fn[0] != (T)0 || Fn[1] != (T)0 || Fn[2] != (T)0) {
+      // u' of the first moment update (old FOL), then cul_equ2 with the new FOL.
+      T Fo[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        Fo[d] = (T)(old_here ? io.fol_old[(int64_t)d * p.n + node] + p.g[d] : p.g[d]);
+      T r1, u1[3];
+      moments<L, T>(p, c, f, Fo, r1, u1);
+      add_forcing<L, T>(f, u1, Fn, (T)p.tau);
+    }
+  }
+  T rho, u[3];
+  moments<L, T>(p, c, f, Fn, rho, u);
+
+  if constexpr (MAT) {
+    if (io.out_f) {
+#pragma unroll
+      for (int k = 0; k < L::Q; ++k) io.out_f[(int64_t)k * p.n + node] = (float)f[k];
+    }
+    if (io.out_u) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) io.out_u[(int64_t)d * p.n + node] = (float)u[d];
+    }
+    if (io.out_rho) io.out_rho[node] = (float)rho;
+    if (io.out_p) io.out_p[node] = (float)(rho / (T)3);
+  } else {
+    collide<L, T>(f, u, rho, (T)p.tau, (T*)nullptr);
+    if (p.has_moving && io.rho_next) {
+      bool mf = false;
+#pragma unroll
+      for (int face = 0; face < 6; ++face)
+        mf |= (p.bc[face] == FUNLBM_BC_WALL_WITH_SPEED) && on_face(p, c, face);
+      if (mf) io.rho_next[node] = (float)rho;
+    }
+    static_for<0, L::Q>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
+      io.dst[(int64_t)K * p.n + node] = S::store(f[K], wr, p.inv_scale, p.scale);
+    });
+  }
+}
+
+}  // namespace funlbm

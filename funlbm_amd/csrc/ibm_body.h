@@ -1,0 +1,425 @@
+// Immersed-boundary bodies shared by the single-lattice kernels (ibm.hip)
+// and the multi-lattice kernels (batch.hip).
+//
+// Per marker (one 64-lane wavefront, lane (i, j, k) = stencil node rl + (i, j, k)):
+//   interp_marker       interpolation + direct forcing
+//   spread_prep_marker  clear the old stencil, claim the owner of each node
+//   spread_write_marker the owner writes w lF lm
+// Per particle (one 256-thread block):
+//   wall_particle       wall spring
+//   forces_particle     force/torque reduction and rigid-body update (double)
+//   pose_particle       rotation and marker positions
+#pragma once
+
+#include "lbm_common.h"
+
+namespace funlbm {
+
+constexpr int kWave = 64;
+constexpr int kIbmBlock = 256;  // 4 markers per block
+constexpr float kPiF = 3.14159265358979323846f;
+
+// Stencil of a marker (src/funlbm/lbm/lbm3d.py:32-44): rl = max(floor(x) - 1, 0),
+// rr = min(rl + 4, size) with size = the config size (node count - 1).
+struct Stencil {
+  int lo[3], hi[3];
+};
+
+__device__ inline Stencil stencil_of(const KParams& p, const float x[3]) {
+  Stencil s;
+  const int size[3] = {p.nx - 1, p.ny - 1, p.nz - 1};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    float rl = floorf(x[d]) - 1.0f;
+    rl = fmaxf(rl, 0.0f);
+    float rr = fminf(rl + 4.0f, (float)size[d]);
+    s.lo[d] = (int)rl;
+    s.hi[d] = (int)rr;
+  }
+  return s;
+}
+
+// Cosine kernel factor (src/funlbm/lbm/lbm3d.py:46-59):
+// phi(r) = (1 + cos(|r pi/2|))/4 where all |r_d| <= 2, else 0.
+__device__ inline float unnormalised_weight(const float x[3], int nx_, int ny_, int nz_) {
+  const float r[3] = {(float)nx_ - x[0], (float)ny_ - x[1], (float)nz_ - x[2]};
+  if (!(fabsf(r[0]) <= 2.0f && fabsf(r[1]) <= 2.0f && fabsf(r[2]) <= 2.0f)) return 0.0f;
+  float t[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) t[d] = (1.0f + cosf(fabsf(r[d] * kPiF / 2.0f))) / 4.0f;
+  return (t[0] * t[1]) * t[2];
+}
+
+template <class T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Lane geometry of a marker: node coordinates and normalised weight.
+struct LaneW {
+  bool valid;
+  int x, y, z;
+  float w;     // normalised weight (0 for invalid lanes)
+  float wsum;  // sum of normalised weights of the stencil
+};
+
+__device__ inline LaneW lane_weight(const KParams& p, const float x[3], int lane) {
+  LaneW lw;
+  const Stencil s = stencil_of(p, x);
+  const int i = lane >> 4, j = (lane >> 2) & 3, k = lane & 3;
+  lw.x = s.lo[0] + i;
+  lw.y = s.lo[1] + j;
+  lw.z = s.lo[2] + k;
+  lw.valid = lw.x < s.hi[0] && lw.y < s.hi[1] && lw.z < s.hi[2];
+  float w = lw.valid ? unnormalised_weight(x, lw.x, lw.y, lw.z) : 0.0f;
+  const float ws = wave_sum(w);
+  if (ws > 0.0f) w = w / ws;
+  lw.w = w;
+  lw.wsum = wave_sum(w);
+  return lw;
+}
+
+// Particle state layout (double[P][FUNLBM_PSTRIDE]).
+__device__ inline double* pst(double* ps, int i) { return ps + (int64_t)i * FUNLBM_PSTRIDE; }
+__device__ inline const double* pst(const double* ps, int i) {
+  return ps + (int64_t)i * FUNLBM_PSTRIDE;
+}
+
+// ---------------------------------------------------------------------------
+// Interpolation + direct forcing (src/funlbm/lbm/lbm3d.py:66-100):
+//   s_u = sum_nodes sum_d u_d w (the three components summed into one scalar),
+//   s_rho = sum w rho, lu = cu + cw x (lx - cx), lF = 2 s_rho (lu - s_u)/dt.
+// FROM_STATE: u', rho' come from pulling the post-stream populations out of
+// the fused state and forming the moments with FOL_old; otherwise from the
+// explicit fields u / rho.  Call with all 64 lanes of the wave.
+// ---------------------------------------------------------------------------
+template <class L, class S, bool FROM_STATE>
+__device__ inline void interp_marker(const KParams& p, int m, int lane,
+                                     const typename S::elem* __restrict__ state,
+                                     const float* __restrict__ fol_old,
+                                     const float* __restrict__ rho_face,
+                                     const float* __restrict__ ufield,
+                                     const float* __restrict__ rhofield,
+                                     const float* __restrict__ lx, const double* __restrict__ P,
+                                     float dt, bool check_sum, float* __restrict__ lu,
+                                     float* __restrict__ lF, int* __restrict__ fail_flag) {
+  using T = typename S::T;
+  const float x[3] = {lx[3 * m], lx[3 * m + 1], lx[3 * m + 2]};
+  const LaneW lw = lane_weight(p, x, lane);
+  float su = 0.0f, sr = 0.0f;
+  if (lw.valid) {
+    const int64_t node = lin(p, lw.x, lw.y, lw.z);
+    float uu[3], rr;
+    if constexpr (FROM_STATE) {
+      const Node c{lw.x, lw.y, lw.z};
+      T f[L::Q];
+      pull_node<L, S>(p, state, c, node, rho_face, f);
+      T F[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        F[d] = (T)(fol_old ? fol_old[(int64_t)d * p.n + node] + p.g[d] : p.g[d]);
+      T r, u[3];
+      moments<L, T>(p, c, f, F, r, u);
+      rr = (float)r;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) uu[d] = (float)u[d];
+    } else {
+      rr = rhofield[node];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) uu[d] = ufield[(int64_t)d * p.n + node];
+    }
+    su = (uu[0] * lw.w + uu[1] * lw.w) + uu[2] * lw.w;
+    sr = rr * lw.w;
+  }
+  su = wave_sum(su);
+  sr = wave_sum(sr);
+  if (lane == 0) {
+    if (check_sum && fabsf(lw.wsum - 1.0f) > 1e-3f) atomicOr(fail_flag, 1);
+    const double d[3] = {(double)x[0] - P[FUNLBM_P_CX], (double)x[1] - P[FUNLBM_P_CX + 1],
+                         (double)x[2] - P[FUNLBM_P_CX + 2]};
+    const double* w = P + FUNLBM_P_CW;
+    const double cr[3] = {w[1] * d[2] - w[2] * d[1], w[2] * d[0] - w[0] * d[2],
+                          w[0] * d[1] - w[1] * d[0]};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const float l = (float)(P[FUNLBM_P_CU + a] + cr[a]);
+      lu[3 * m + a] = l;
+      lF[3 * m + a] = ((2.0f * sr) * (l - su)) / dt;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Wall spring (src/funlbm/lbm/lbm3d.py:121-167) for the markers [b, e) of one
+// particle.  Extreme markers are the first arg-min / arg-max per axis (torch's
+// tie rule); d = max(k0 (1 - |coord - bound|/(2 dx)), 0), bounds
+// [0,0,0,size]; the six rows lF[ext_j] + n_j d_j are gathered before any is
+// written, then written in order j = 0..5 so a duplicate index keeps the last
+// row.  Call with the whole block.
+// ---------------------------------------------------------------------------
+__device__ inline void wall_particle(const KParams& p, int b, int e, const float* __restrict__ lx,
+                                     float* __restrict__ lF, float dx) {
+  __shared__ float sv[6][kIbmBlock];
+  __shared__ int si[6][kIbmBlock];
+  float bv[6];
+  int bi[6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    bv[j] = (j < 3) ? INFINITY : -INFINITY;
+    bi[j] = 0x7fffffff;
+  }
+  for (int m = b + threadIdx.x; m < e; m += kIbmBlock) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const float v = lx[3 * m + (j % 3)];
+      const bool better = (j < 3) ? (v < bv[j] || (v == bv[j] && m < bi[j]))
+                                  : (v > bv[j] || (v == bv[j] && m < bi[j]));
+      if (better) {
+        bv[j] = v;
+        bi[j] = m;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    sv[j][threadIdx.x] = bv[j];
+    si[j][threadIdx.x] = bi[j];
+  }
+  __syncthreads();
+  for (int s = kIbmBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        const float v = sv[j][threadIdx.x + s];
+        const int i = si[j][threadIdx.x + s];
+        const float cv = sv[j][threadIdx.x];
+        const int ci = si[j][threadIdx.x];
+        const bool better =
+            (j < 3) ? (v < cv || (v == cv && i < ci)) : (v > cv || (v == cv && i < ci));
+        if (better) {
+          sv[j][threadIdx.x] = v;
+          si[j][threadIdx.x] = i;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && e > b) {
+    const float bounds[6] = {0.0f, 0.0f, 0.0f, (float)(p.nx - 1), (float)(p.ny - 1), (float)(p.nz - 1)};
+    const float normals[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+    float dist[6];
+    float dmax = 0.0f;
+    int ext[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      ext[j] = si[j][0];
+      const float c = lx[3 * ext[j] + (j % 3)];
+      float d = 300.0f * (1.0f - fabsf(c - bounds[j]) / (2.0f * dx));
+      d = fmaxf(d, 0.0f);
+      dist[j] = d;
+      dmax = fmaxf(dmax, d);
+    }
+    if (dmax != 0.0f) {
+      float rows[6][3];
+#pragma unroll
+      for (int j = 0; j < 6; ++j)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) rows[j][a] = lF[3 * ext[j] + a] + normals[j][a] * dist[j];
+#pragma unroll
+      for (int j = 0; j < 6; ++j)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) lF[3 * ext[j] + a] = rows[j][a];
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Rigid-body update (src/funlbm/particle/base/base.py:120-151) in double:
+// cF = sum(-lF lm) + (1 - rho_p/rho_f) V g; cu += cF/V dt; cx += cu dt;
+// cT = -sum((lx - cx) x lF) lm; cw += 0.1 cT dt / I.  Fixed-order tree
+// reductions over the block.  Call with the whole block.
+// ---------------------------------------------------------------------------
+__device__ inline void forces_particle(int b, int e, const float* __restrict__ lx,
+                                       const float* __restrict__ lF, const float* __restrict__ lm,
+                                       double* __restrict__ P, double dt, double rho_f) {
+  __shared__ double red[6][kIbmBlock];
+  __shared__ double cxn[3];
+  double fs[3] = {0, 0, 0};
+  for (int m = b + threadIdx.x; m < e; m += kIbmBlock)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) fs[a] += -(double)lF[3 * m + a] * (double)lm[m];
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 3; ++a) red[a][threadIdx.x] = fs[a];
+  __syncthreads();
+  for (int s = kIbmBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) red[a][threadIdx.x] += red[a][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double V = P[FUNLBM_P_MASS];
+    const double buoy = (1.0 - P[FUNLBM_P_RHO] / rho_f) * V;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double F = red[a][0] + buoy * P[FUNLBM_P_GRAV + a];
+      P[FUNLBM_P_CF + a] = F;
+      P[FUNLBM_P_CU + a] = P[FUNLBM_P_CU + a] + F / V * dt;
+      P[FUNLBM_P_CX + a] = P[FUNLBM_P_CX + a] + P[FUNLBM_P_CU + a] * dt;
+      cxn[a] = P[FUNLBM_P_CX + a];
+    }
+  }
+  __syncthreads();
+  double ts[3] = {0, 0, 0};
+  for (int m = b + threadIdx.x; m < e; m += kIbmBlock) {
+    const double r[3] = {(double)lx[3 * m] - cxn[0], (double)lx[3 * m + 1] - cxn[1],
+                         (double)lx[3 * m + 2] - cxn[2]};
+    const double f[3] = {lF[3 * m], lF[3 * m + 1], lF[3 * m + 2]};
+    const double w = lm[m];
+    ts[0] += (r[1] * f[2] - r[2] * f[1]) * w;
+    ts[1] += (r[2] * f[0] - r[0] * f[2]) * w;
+    ts[2] += (r[0] * f[1] - r[1] * f[0]) * w;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) red[3 + a][threadIdx.x] = ts[a];
+  __syncthreads();
+  for (int s = kIbmBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) red[3 + a][threadIdx.x] += red[3 + a][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double T = -red[3 + a][0];
+      P[FUNLBM_P_CT + a] = T;
+      P[FUNLBM_P_CW + a] = P[FUNLBM_P_CW + a] + 0.1 * T * dt / P[FUNLBM_P_INERTIA + a];
+    }
+  }
+  __syncthreads();
+}
+
+// Rotation matrix of a rotation vector (Rodrigues), double.
+__device__ inline void rotvec_matrix(const double v[3], double R[9]) {
+  const double th2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  const double th = sqrt(th2);
+  double s, c1;  // sin(th)/th and (1 - cos(th))/th^2
+  if (th < 1e-4) {
+    s = 1.0 - th2 / 6.0 + th2 * th2 / 120.0;
+    c1 = 0.5 - th2 / 24.0 + th2 * th2 / 720.0;
+  } else {
+    s = sin(th) / th;
+    c1 = (1.0 - cos(th)) / th2;
+  }
+  const double x = v[0], y = v[1], z = v[2];
+  R[0] = 1.0 - c1 * (y * y + z * z);
+  R[1] = -s * z + c1 * x * y;
+  R[2] = s * y + c1 * x * z;
+  R[3] = s * z + c1 * x * y;
+  R[4] = 1.0 - c1 * (x * x + z * z);
+  R[5] = -s * x + c1 * y * z;
+  R[6] = -s * y + c1 * x * z;
+  R[7] = s * x + c1 * y * z;
+  R[8] = 1.0 - c1 * (x * x + y * y);
+}
+
+// Pose update (src/funlbm/particle/base/base.py:153-156,
+// src/funlbm/particle/coord.py:73-108): angle += cw (no dt), R = rotvec(angle),
+// lx = float32(R lag) + float32(cx).  Call with the whole block.
+__device__ inline void pose_particle(int b, int e, const float* __restrict__ lag,
+                                     float* __restrict__ lx, double* __restrict__ P, bool advance) {
+  __shared__ double R[9];
+  __shared__ float cx[3];
+  if (threadIdx.x == 0) {
+    double v[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (advance) P[FUNLBM_P_ANGLE + a] = P[FUNLBM_P_ANGLE + a] + P[FUNLBM_P_CW + a];
+      v[a] = P[FUNLBM_P_ANGLE + a];
+      cx[a] = (float)P[FUNLBM_P_CX + a];
+    }
+    double r[9];
+    rotvec_matrix(v, r);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      R[i] = r[i];
+      P[FUNLBM_P_ROT + i] = r[i];
+    }
+  }
+  __syncthreads();
+  for (int m = b + threadIdx.x; m < e; m += kIbmBlock) {
+    const double q[3] = {lag[3 * m], lag[3 * m + 1], lag[3 * m + 2]};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const double v = R[3 * a] * q[0] + R[3 * a + 1] * q[1] + R[3 * a + 2] * q[2];
+      lx[3 * m + a] = (float)v + cx[a];
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Spreading (src/funlbm/lbm/lbm3d.py:102-119): FOL := 0, then for every marker
+// in order FOL[stencil] = w lF lm, overwriting.  A node ends up with the value
+// of the highest-numbered marker whose stencil box covers it.
+//   prep   zero the stencil this buffer received from the marker last time
+//          and raise owner[node] to (stamp, marker) with an integer atomicMax
+//   write  the owning marker of each node writes w lF lm and stamps the tile
+// The stamp advances by one per spreading, so owner[] never needs clearing.
+// ---------------------------------------------------------------------------
+__device__ inline void spread_prep_marker(const KParams& p, int m, int key_id, int lane,
+                                          const float* __restrict__ lx,
+                                          const int* __restrict__ clear_origin,
+                                          float* __restrict__ fol,
+                                          unsigned long long* __restrict__ owner, uint32_t stamp) {
+  const int i = lane >> 4, j = (lane >> 2) & 3, k = lane & 3;
+  if (clear_origin) {
+    const int ox = clear_origin[3 * m] + i, oy = clear_origin[3 * m + 1] + j,
+              oz = clear_origin[3 * m + 2] + k;
+    if (ox < p.nx && oy < p.ny && oz < p.nz) {
+      const int64_t node = lin(p, ox, oy, oz);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) fol[(int64_t)d * p.n + node] = 0.0f;
+    }
+  }
+  const float x[3] = {lx[3 * m], lx[3 * m + 1], lx[3 * m + 2]};
+  const Stencil s = stencil_of(p, x);
+  const int nx_ = s.lo[0] + i, ny_ = s.lo[1] + j, nz_ = s.lo[2] + k;
+  if (nx_ < s.hi[0] && ny_ < s.hi[1] && nz_ < s.hi[2]) {
+    const unsigned long long key = ((unsigned long long)stamp << 32) | (unsigned long long)(unsigned)key_id;
+    atomicMax(&owner[lin(p, nx_, ny_, nz_)], key);
+  }
+}
+
+__device__ inline void spread_write_marker(const KParams& p, int m, int key_id, int lane,
+                                           const float* __restrict__ lx,
+                                           const float* __restrict__ lF,
+                                           const float* __restrict__ lm, float* __restrict__ fol,
+                                           const unsigned long long* __restrict__ owner,
+                                           int* __restrict__ origin, uint32_t* __restrict__ tiles,
+                                           uint32_t stamp) {
+  const float x[3] = {lx[3 * m], lx[3 * m + 1], lx[3 * m + 2]};
+  const LaneW lw = lane_weight(p, x, lane);
+  if (lane == 0 && origin) {
+    const Stencil s = stencil_of(p, x);
+    origin[3 * m] = s.lo[0];
+    origin[3 * m + 1] = s.lo[1];
+    origin[3 * m + 2] = s.lo[2];
+  }
+  if (!lw.valid) return;
+  const int64_t node = lin(p, lw.x, lw.y, lw.z);
+  const unsigned long long key = ((unsigned long long)stamp << 32) | (unsigned long long)(unsigned)key_id;
+  if (owner[node] != key) return;
+  const float mm = lm[m];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) fol[(int64_t)d * p.n + node] = (lw.w * lF[3 * m + d]) * mm;
+  if (tiles) tiles[(uint32_t)node >> 8] = stamp;
+}
+
+}  // namespace funlbm

@@ -139,7 +139,8 @@ struct StoreBF16 {
 // ---------------------------------------------------------------------------
 struct KParams {
   int nx, ny, nz;
-  int64_t n;  // nodes
+  int sxy;    // ny * nz, the x stride of the node index
+  int64_t n;  // nodes (< 2^31, checked on the host)
   int bc[6];
   int uzero;
   int has_moving;  // any WALL_WITH_SPEED face
@@ -177,11 +178,17 @@ struct Node {
 };
 __device__ inline Node node_coords(const KParams& p, int64_t node) {
   Node c;
-  int64_t t = node / p.nz;
-  c.z = (int)(node - t * p.nz);
-  c.y = (int)(t % p.ny);
-  c.x = (int)(t / p.ny);
+  const uint32_t nd = (uint32_t)node;
+  const uint32_t t = nd / (uint32_t)p.nz;
+  c.z = (int)(nd - t * (uint32_t)p.nz);
+  const uint32_t x = t / (uint32_t)p.ny;
+  c.y = (int)(t - x * (uint32_t)p.ny);
+  c.x = (int)x;
   return c;
+}
+
+__device__ inline bool interior_node(const KParams& p, const Node& c) {
+  return (c.x > 0) & (c.x < p.nx - 1) & (c.y > 0) & (c.y < p.ny - 1) & (c.z > 0) & (c.z < p.nz - 1);
 }
 
 __device__ inline bool on_face(const KParams& p, const Node& c, int face) {
@@ -253,7 +260,7 @@ __device__ inline Src source_of(const KParams& p, const Node& c) {
 }
 
 __device__ inline int64_t lin(const KParams& p, int x, int y, int z) {
-  return ((int64_t)x * p.ny + y) * p.nz + z;
+  return (int64_t)(x * p.sxy + y * p.nz + z);
 }
 
 // Moving-wall correction 2 (rho w_k) (e_k . uw) / cs^2 in the reference's
@@ -270,11 +277,24 @@ __device__ inline T moving_wall_term(const KParams& p, int face, T rho) {
 }
 
 // Pull all Q post-stream populations of node c out of a state buffer.
+// Interior nodes take the plain shifted source node - e_k; face nodes go
+// through the full override logic of source_of.
 template <class L, class S>
 __device__ inline void pull_node(const KParams& p, const typename S::elem* __restrict__ src,
                                  const Node& c, int64_t node, const float* __restrict__ rho_prev,
                                  typename S::T* f) {
   using T = typename S::T;
+  if (interior_node(p, c)) {
+    const int nd = (int)node;
+    static_for<0, L::Q>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      constexpr int ex = L::E[K][0], ey = L::E[K][1], ez = L::E[K][2];
+      const int sn = nd - (ex * p.sxy + ey * p.nz + ez);
+      f[K] = S::load(src[(int64_t)K * p.n + sn], (float)(wt<float>(L::W[K]) * p.rho0), p.inv_scale,
+                     p.scale);
+    });
+    return;
+  }
   static_for<0, L::Q>([&](auto kc) {
     constexpr int K = decltype(kc)::value;
     const Src s = source_of<L, K>(p, c);

@@ -112,6 +112,8 @@ class FlowD3(Flow):
         self._fol = [torch.zeros((3, N), device=dev, dtype=_F32), None]
         self._fc = 0
         self._fol_live = [False, False]
+        self._fol_tiled = [False, False]
+        self._tiles = (None, None)
         self._state = [None, None]
         self._cur = 0
         self._rho_face = [None, None]
@@ -230,32 +232,42 @@ class FlowD3(Flow):
             self._fol[j] = torch.zeros((3, self.n), device=self.device, dtype=_F32)
         return self._fol[j]
 
-    def publish_fol(self, written: bool):
+    def publish_fol(self, written: bool, tiled: bool = False):
         """Makes FOL_new the current FOL.  Without a spread it is zero, as the
         reference's lagrange_to_flow zeroes FOL every step
-        (src/funlbm/lbm/lbm3d.py:104)."""
+        (src/funlbm/lbm/lbm3d.py:104).  `tiled`: the spreading recorded its
+        support in the tile table given to finish_step."""
         j = 1 - self._fc
         if written:
             self._fol_live[j] = True
         elif self._fol_live[j]:
             self._fol[j].zero_()
             self._fol_live[j] = False
+        self._fol_tiled[j] = bool(written and tiled)
         self._fc = j
 
-    def finish_step(self, fol_new_written: bool):
+    def finish_step(self, fol_new_written: bool, tiles=None, ctl=None):
         """The fused kernel: stream, forcing with FOL_new, moments, collide
-        (src/funlbm/lbm/base.py:162-175 and the next step's :161)."""
+        (src/funlbm/lbm/base.py:162-175 and the next step's :161).  tiles/ctl:
+        the spreading's tile table, which limits the FOL reads to its support
+        (used when both FOL buffers come from consecutive spreadings)."""
         if self._mode != "fused":
             raise RuntimeError("finish_step without begin_step")
         a, b = self._cur, 1 - self._cur
         fol_old = self._fol_ptr(self._fc)
         fol_new = nat.ptr(self._fol[1 - self._fc]) if fol_new_written else None
+        use_tiles = (
+            tiles is not None and fol_new_written
+            and (fol_old is None or self._fol_tiled[self._fc])
+        )
+        self._tiles = (tiles, ctl) if use_tiles else (None, None)
         nat.call(
             "funlbm_fused_step", self.lat, nat.ptr(self._state[a]), nat.ptr(self._state[b]),
-            fol_old, fol_new, nat.ptr(self._rho_face[a]), nat.ptr(self._rho_face[b]), self._stream(),
+            fol_old, fol_new, nat.ptr(self._tiles[0]), nat.ptr(self._tiles[1]),
+            nat.ptr(self._rho_face[a]), nat.ptr(self._rho_face[b]), self._stream(),
         )
         self._cur = b
-        self.publish_fol(fol_new_written)
+        self.publish_fol(fol_new_written, tiled=use_tiles)
         self._mat_valid = False
 
     def _materialize(self):
@@ -265,8 +277,9 @@ class FlowD3(Flow):
         a = 1 - self._cur  # state streamed by the step just taken
         nat.call(
             "funlbm_materialize", self.lat, nat.ptr(self._state[a]),
-            self._fol_ptr(1 - self._fc), self._fol_ptr(self._fc), nat.ptr(self._rho_face[a]),
-            nat.ptr(self._f), nat.ptr(self._u), nat.ptr(self._rho), nat.ptr(self._p), self._stream(),
+            self._fol_ptr(1 - self._fc), self._fol_ptr(self._fc), nat.ptr(self._tiles[0]),
+            nat.ptr(self._tiles[1]), nat.ptr(self._rho_face[a]), nat.ptr(self._f), nat.ptr(self._u),
+            nat.ptr(self._rho), nat.ptr(self._p), self._stream(),
         )
         self._mat_valid = True
 
@@ -312,6 +325,7 @@ class FlowD3(Flow):
     def FOL(self):
         # the caller may write through the view
         self._fol_live[self._fc] = True
+        self._fol_tiled[self._fc] = False
         self._pristine = False
         return self._view(self._fol[self._fc], 3)
 
@@ -320,6 +334,7 @@ class FlowD3(Flow):
         v = torch.as_tensor(value, device=self.device, dtype=_F32).reshape(*self.nodes, 3)
         self._fol[self._fc].copy_(v.permute(3, 0, 1, 2).reshape(3, self.n))
         self._fol_live[self._fc] = True
+        self._fol_tiled[self._fc] = False
         self._pristine = False
 
     @property

@@ -33,6 +33,7 @@ class LBMD3(LBMBase):
                 torch.zeros((sw.n_markers, 3), device=dev, dtype=torch.int32) for _ in range(2)
             ]
             self._origin_ok = [False, False]
+            self._tiles = torch.zeros((self.flow.n + 255) // 256, device=dev, dtype=torch.int32)
 
     # ------------------------------------------------------------------
     # IBM phases
@@ -75,8 +76,8 @@ class LBMD3(LBMBase):
         clear = nat.ptr(self._origin[j]) if (fl._fol_live[j] and self._origin_ok[j]) else None
         nat.call(
             "funlbm_ibm_spread", fl.lat32, sw.n_markers, nat.ptr(sw.lx), nat.ptr(sw.lF), nat.ptr(sw.lm),
-            nat.ptr(buf), clear, nat.ptr(self._origin[j]), nat.ptr(self._owner), nat.ptr(sw.ctl),
-            fl._stream(),
+            nat.ptr(buf), clear, nat.ptr(self._origin[j]), nat.ptr(self._owner), nat.ptr(self._tiles),
+            nat.ptr(sw.ctl), fl._stream(),
         )
         self._origin_ok[j] = True
 
@@ -84,7 +85,7 @@ class LBMD3(LBMBase):
         """Spreading into a zeroed FOL (src/funlbm/lbm/lbm3d.py:102-119)."""
         if self._has_particles:
             self._spread()
-        self.flow.publish_fol(self._has_particles)
+        self.flow.publish_fol(self._has_particles, tiled=self._has_particles)
 
     def _run_step_fused(self):
         fl, sw = self.flow, self.particle_swarm
@@ -94,9 +95,11 @@ class LBMD3(LBMBase):
             self.particle_to_wall()
             sw.forces(self.config.dt, 1.0)
             self._spread()
-        fl.finish_step(self._has_particles)
         if self._has_particles:
+            fl.finish_step(True, tiles=self._tiles, ctl=sw.ctl)
             sw.pose()
+        else:
+            fl.finish_step(False)
 
     def _handle_immersed_boundary(self) -> None:
         """Explicit path: the reference's sequence (src/funlbm/lbm/base.py:165-175)."""

@@ -119,17 +119,21 @@ int funlbm_equilibrium_state(const funlbm_lattice* lat, const float* rho, const 
  * pull-stream state_src with the face overrides, where FOL_new != 0 add the
  * forcing term with u' from FOL_old, take moments with FOL_new, zero u on the
  * masked faces, collide, store into state_dst.  fol_old / fol_new ([3][x][y][z]
- * float) may be NULL (zero).  rho_prev / rho_next are used only with
- * WALL_WITH_SPEED faces (the density of the previous step on those faces). */
+ * float) may be NULL (zero).  fol_tiles (uint32 per 256-node tile, written by
+ * funlbm_ibm_spread) with ctl limits the FOL reads to the tiles whose entry
+ * equals ctl[0] (the last spreading's stamp); NULL reads FOL everywhere.
+ * rho_prev / rho_next are used only with WALL_WITH_SPEED faces (the density
+ * of the previous step on those faces). */
 int funlbm_fused_step(const funlbm_lattice* lat, const void* state_src, void* state_dst,
-                      const float* fol_old, const float* fol_new, const float* rho_prev,
-                      float* rho_next, void* stream);
+                      const float* fol_old, const float* fol_new, const uint32_t* fol_tiles,
+                      const int32_t* ctl, const float* rho_prev, float* rho_next, void* stream);
 
 /* The same pass as funlbm_fused_step without the collision: writes the
  * reference's f, u, rho and p (float) after the step.  Any output may be NULL. */
 int funlbm_materialize(const funlbm_lattice* lat, const void* state_src, const float* fol_old,
-                       const float* fol_new, const float* rho_prev, float* f, float* u,
-                       float* rho, float* p, void* stream);
+                       const float* fol_new, const uint32_t* fol_tiles, const int32_t* ctl,
+                       const float* rho_prev, float* f, float* u, float* rho, float* p,
+                       void* stream);
 
 /* ---- immersed boundary and rigid particles ----
  *
@@ -190,10 +194,49 @@ int funlbm_particle_pose(int32_t n_particles, const int32_t* marker_offsets, con
  * overwriting.  The buffer must be zero outside the stencils whose origins
  * (int32[M][3]) are given in clear_origin (NULL: zero everywhere); the origins
  * of this call are written to `origin`.  owner is uint64[nodes] device memory,
- * zeroed once before first use and never again. */
+ * zeroed once before first use and never again.  fol_tiles (uint32 per
+ * 256-node tile, may be NULL) receives this call's stamp in every tile it
+ * writes; ctl[0] holds that stamp after the call. */
 int funlbm_ibm_spread(const funlbm_lattice* lat, int32_t n_markers, const float* lx, const float* lF,
                       const float* lm, float* fol, const int32_t* clear_origin, int32_t* origin,
-                      uint64_t* owner, int32_t* ctl, void* stream);
+                      uint64_t* owner, uint32_t* fol_tiles, int32_t* ctl, void* stream);
+
+/* ---- many lattices per launch ----
+ *
+ * Independent cases of uneven size share pools: two state pools (the case's
+ * [q][n] block at element offset f_offset in each), two FOL pools ([3][n] at
+ * 3*node_offset), an owner pool (uint64, n at node_offset) and a tile table
+ * (uint32, ceil(n/256) at tile_begin).  A case descriptor is an opaque block
+ * of funlbm_case_bytes() bytes built on the host by funlbm_case_pack and
+ * copied by the caller into a device array indexed by case slot.  No
+ * reference counterpart: the reference solves one lattice per process. */
+int funlbm_case_bytes(void);
+
+/* Fills the descriptor of one case.  PERIODICAL and WALL faces only. */
+int funlbm_case_pack(const funlbm_lattice* lat, int64_t f_offset, int64_t node_offset,
+                     int32_t tile_begin, void* desc_out);
+
+/* One timestep of every case whose steps-left counter is positive: the
+ * single-lattice sequence (interp, wall, forces, spreading, fused step, pose)
+ * as persistent kernels over device-side work lists, then the per-case
+ * counters advance (parity flips, left -1, done +1) and hdr[4] (the spreading
+ * stamp) +1.  hdr is int32[8] device memory: [0] active tiles in
+ * tile_case/tile_local, [1] marker slots, [2] particle slots, [3] case slots,
+ * [4] stamp.  A slot whose case index is -1 is skipped.  particle_markers holds
+ * [begin, end) marker ranges (two int32 per particle).  traj (may be NULL)
+ * receives float[4] = (cx, cy, cz, step) per particle and step in a ring of
+ * traj_cap[0] entries.  The grid sizes fix the persistent launch shapes. */
+int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, int32_t* case_par,
+                      int32_t* case_left, int32_t* case_done, int32_t* case_flag,
+                      const int32_t* tile_case, const int32_t* tile_local,
+                      const int32_t* marker_case, const int32_t* marker_particle,
+                      const int32_t* case_marker_begin, const int32_t* particle_case,
+                      const int32_t* particle_markers, void* pool_a, void* pool_b, float* fol_a,
+                      float* fol_b, uint64_t* owner, uint32_t* fol_tiles, int32_t* origin_a,
+                      int32_t* origin_b, int32_t* origin_ok, const float* lag, float* lx, float* lu,
+                      float* lF, const float* lm, double* pstate, float* traj,
+                      const int32_t* traj_cap, int32_t grid_tiles, int32_t grid_markers,
+                      int32_t grid_particles, int32_t with_ibm, void* stream);
 
 /* Plain device copy of n_bytes (16-byte aligned), used to calibrate the
  * measured copy bandwidth.  No reference counterpart. */

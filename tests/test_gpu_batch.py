@@ -1,0 +1,124 @@
+"""The multi-lattice engine (needs a GPU): each case equals its own
+single-lattice solve bit for bit, alone or in a mix, eager or replayed from a
+HIP graph."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from funlbm_amd.batch.engine import BatchEngine, CaseSpec, ParticleSpec
+
+pytestmark = pytest.mark.gpu
+
+
+def sphere_case(nodes, q=19, tau=0.6, r=4.0, rou=1.5, steps=30, center=None):
+    center = center or tuple(n / 2 for n in nodes)
+    return CaseSpec(nodes=nodes, q=q, tau=tau, steps=steps,
+                    particles=[ParticleSpec(kind="sphere", r=r, rou=rou, center=center)],
+                    gravity=(0.0, 0.0, -2e-5))
+
+
+def lbmd3_for(spec, storage):
+    from funlbm_amd.lbm import LBMD3, Config
+
+    faces = {f: {"code": "WALL"} for f in ("input", "output", "back", "forward", "bottom", "top")}
+    p = spec.particles[0]
+    cfg = Config().from_json({
+        "flow": {"param_type": f"D3Q{spec.q}", "size": [n - 1 for n in spec.nodes],
+                 "param": {"mu": (float(np.float32(spec.tau)) - 0.5) / 3, "gravity": list(spec.gravity)},
+                 "boundary": faces, "storage": storage},
+        "particles": [{"type": "sphere", "r": p.r, "rou": p.rou,
+                       "coord": {"center": list(p.center)}}],
+        "log_every": 0,
+    })
+    lbm = LBMD3(config=cfg)
+    lbm.init()
+    return lbm
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_batch_case_equals_single_lattice(storage):
+    spec = sphere_case((20, 22, 26), steps=25)
+    lbm = lbmd3_for(spec, storage)
+    spec.tau = lbm.flow.lat.tau  # the solver's tau comes from mu; use the same float32
+    eng = BatchEngine(storage=storage, max_nodes=1 << 16, max_markers=4096, max_particles=8, max_cases=4)
+    cid = eng.add(spec)
+    eng.step(25)
+    assert lbm.flow.lat.tau == eng.slots[cid].lat.tau
+    for s in range(1, 26):
+        lbm.step = s
+        lbm.run_step()
+    u, rho = eng.fields(cid)
+    assert torch.equal(eng.particle_rows(cid)[0, :15], lbm.particle_swarm.pstate[0, :15])
+    assert torch.equal(u, lbm.flow.u)
+    assert torch.equal(rho, lbm.flow.rou)
+
+
+def _mixed_specs():
+    return [
+        sphere_case((24, 20, 30), q=19, tau=0.62, r=4.0, steps=40),
+        CaseSpec(nodes=(18, 18, 18), q=15, tau=0.8, steps=40,
+                 particles=[ParticleSpec(kind="sphere", r=3.0, rou=2.0, center=(9, 9, 10))]),
+        CaseSpec(nodes=(33, 17, 21), q=27, tau=0.55, steps=40,
+                 particles=[ParticleSpec(kind="ellipsoid", abc=(3, 3, 5), rou=1.3, center=(16, 8, 10),
+                                         angle=(0.3, 0.2, 0.1)),
+                            ParticleSpec(kind="sphere", r=2.5, rou=2.5, center=(8, 8, 12))]),
+        CaseSpec(nodes=(16, 16, 40), q=19, tau=0.9, steps=40, particles=[]),
+    ]
+
+
+def test_alone_equals_mixed_bitwise():
+    specs = _mixed_specs()
+    mix = BatchEngine(storage="f16", max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8)
+    ids = [mix.add(s) for s in specs]
+    mix.step(40)
+    for s, cid in zip(specs, ids):
+        alone = BatchEngine(storage="f16", max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8)
+        a = alone.add(s)
+        alone.step(40)
+        ua, ra = alone.fields(a)
+        um, rm = mix.fields(cid)
+        assert torch.equal(ua, um) and torch.equal(ra, rm)
+        if s.particles:
+            assert torch.equal(alone.particle_rows(a)[:, :15], mix.particle_rows(cid)[:, :15])
+
+
+def test_graph_replay_equals_eager_and_step_counts():
+    specs = _mixed_specs()
+    specs[1].steps = 5
+    specs[3].steps = 9
+    eager = BatchEngine(storage="f32", max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8)
+    graph = BatchEngine(storage="f32", max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8)
+    ie = [eager.add(s) for s in specs]
+    ig = [graph.add(s) for s in specs]
+    eager.step(12)
+    graph.capture(4)
+    for _ in range(3):
+        graph.run_graph()
+        torch.cuda.synchronize()
+    done_e = eager.case_done.cpu().numpy()
+    done_g = graph.case_done.cpu().numpy()
+    assert [done_e[i] for i in ie] == [12, 5, 12, 9]
+    assert [done_g[i] for i in ig] == [12, 5, 12, 9]
+    for a, b in zip(ie, ig):
+        ua, ra = eager.fields(a)
+        ub, rb = graph.fields(b)
+        assert torch.equal(ua, ub) and torch.equal(ra, rb)
+
+
+def test_retire_and_reuse_slot():
+    specs = _mixed_specs()
+    eng = BatchEngine(storage="f16", max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8)
+    first = eng.add(specs[0])
+    eng.step(40)
+    assert eng.finished() == [first]
+    eng.remove(first)
+    again = eng.add(specs[2])
+    eng.step(40)
+    fresh = BatchEngine(storage="f16", max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8)
+    f = fresh.add(specs[2])
+    fresh.step(40)
+    assert torch.equal(eng.fields(again)[0], fresh.fields(f)[0])
+    assert torch.equal(eng.particle_rows(again)[:, :15], fresh.particle_rows(f)[:, :15])

@@ -1,0 +1,133 @@
+"""Parity of the immersed-boundary path (HIP) with the oracle (needs a GPU).
+
+The oracle keeps the particle state in double here, as the build does, so
+the comparison isolates the kernels; the float32 reference state is covered
+by the centroid tolerance test at the end."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import ibm_oracle as ibo
+from oracle import lbm_oracle as orc
+from tests.helpers import flow_json, rel_linf
+
+pytestmark = pytest.mark.gpu
+
+W, P = orc.WALL, orc.PERIODICAL
+
+
+def make_case(q=19, size=(23, 23, 31), mu=1 / 30, bcs=(W,) * 6, particles=None, gl=2e-5, fused=True,
+              extra_flow=None):
+    from funlbm_amd.lbm import LBMD3, Config
+
+    flow = flow_json(q, size, mu, list(bcs))
+    flow["param"]["gl"] = gl
+    if extra_flow:
+        flow.update(extra_flow)
+    cfg = Config().from_json({"flow": flow, "particles": particles or [], "log_every": 0, "fused": fused})
+    lbm = LBMD3(config=cfg)
+    lbm.init()
+    return lbm
+
+
+def oracle_case(lbm, q=19, size=(23, 23, 31), mu=1 / 30, bcs=(W,) * 6, pdtype=np.float64, gravity=None):
+    nodes = tuple(s + 1 for s in size)
+    st = orc.init_flow(q, nodes, mu=mu, bc=list(bcs))
+    orc.init_solver(st)
+    parts = []
+    for p in lbm.particle_swarm.particles:
+        kind = "ellipsoid" if p.config.type == "ellipsoid" else "sphere"
+        c = p.config.coord_config
+        kw = dict(r=p.r) if kind == "sphere" else dict(abc=(p.ra, p.rb, p.rc))
+        parts.append(ibo.Particle(kind=kind, rou=p.rou, center=c.center, angle=(c.alpha, c.beta, c.gamma),
+                                  pdtype=pdtype, lagrange=p._lagrange, gravity=gravity, **kw))
+    return st, parts
+
+
+SPHERE = [{"type": "sphere", "r": 4, "rou": 1.5, "coord": {"center": [12, 12, 20]}}]
+
+
+def step_both(lbm, st, parts, size, steps, gl):
+    for s in range(1, steps + 1):
+        lbm.step = s
+        lbm.run_step()
+        ibo.run_step(st, parts, size, dt=1.0, gl=gl, step=s)
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_sphere_matches_oracle(fused):
+    size = (23, 23, 31)
+    lbm = make_case(particles=SPHERE, fused=fused)
+    st, parts = oracle_case(lbm)
+    step_both(lbm, st, parts, list(size), 30, 2e-5)
+    p, o = lbm.particle_swarm.particles[0], parts[0]
+    assert np.allclose(p.cx.cpu().numpy(), o.cx, rtol=0, atol=1e-6)
+    assert np.allclose(p.cu.cpu().numpy(), o.cu, rtol=1e-4, atol=1e-9)
+    assert np.allclose(p.cw.cpu().numpy(), o.cw, rtol=1e-3, atol=1e-9)
+    assert np.max(np.abs(p.lx.cpu().numpy() - o.lx)) < 1e-5
+    assert rel_linf(lbm.flow.f.cpu().numpy(), st.f) < 1e-5
+    assert rel_linf(lbm.flow.FOL.cpu().numpy(), st.FOL) < 1e-4
+
+
+def test_fused_and_explicit_ibm_paths_agree_bitwise():
+    a = make_case(particles=SPHERE, fused=True)
+    b = make_case(particles=SPHERE, fused=False)
+    for s in range(1, 16):
+        a.step = b.step = s
+        a.run_step()
+        b.run_step()
+    assert torch.equal(a.particle_swarm.pstate, b.particle_swarm.pstate)
+    assert torch.equal(a.particle_swarm.lx, b.particle_swarm.lx)
+    assert torch.equal(a.flow.f, b.flow.f)
+
+
+def test_two_ellipsoids_periodic_box_overwrite_order():
+    """Two ellipsoids whose stencils touch: the spreading keeps the
+    reference's marker order (particle-major, last writer wins)."""
+    size = (27, 23, 23)
+    bcs = (P,) * 6
+    parts = [
+        {"type": "ellipsoid", "a": 3, "b": 3, "c": 5, "rou": 1.2, "coord": {"center": [10, 12, 12]}},
+        {"type": "ellipsoid", "a": 3, "b": 3, "c": 5, "rou": 1.2,
+         "coord": {"center": [16.5, 12, 12], "alpha": 0.3, "beta": 0.2, "gamma": 0.1}},
+    ]
+    lbm = make_case(q=27, size=size, mu=0.05 / 3, bcs=bcs, particles=parts, gl=1e-5)
+    st, ops = oracle_case(lbm, q=27, size=size, mu=0.05 / 3, bcs=bcs)
+    step_both(lbm, st, ops, list(size), 12, 1e-5)
+    for p, o in zip(lbm.particle_swarm.particles, ops):
+        assert np.allclose(p.cx.cpu().numpy(), o.cx, atol=1e-6)
+        assert np.max(np.abs(p.lx.cpu().numpy() - o.lx)) < 1e-5
+    assert rel_linf(lbm.flow.FOL.cpu().numpy(), st.FOL) < 1e-4
+    assert rel_linf(lbm.flow.f.cpu().numpy(), st.f) < 1e-5
+
+
+def test_gravity_vector_and_float32_reference_centroid():
+    """Gravity along -z through the gravity extension; the build's double
+    particle state stays within 1e-4 lattice units of the oracle's float32
+    (reference) particle state over 100 steps."""
+    size = (23, 23, 31)
+    g = [0.0, 0.0, -2e-5]
+    lbm = make_case(particles=SPHERE, extra_flow={"gravity": g})
+    st, parts = oracle_case(lbm, pdtype=np.float32, gravity=g)
+    step_both(lbm, st, parts, list(size), 100, 0.0)
+    p, o = lbm.particle_swarm.particles[0], parts[0]
+    assert np.max(np.abs(p.cx.cpu().numpy() - o.cx.astype(np.float64))) < 1e-4
+    assert p.cx[2].item() < 20.0  # falls along -z
+
+
+def test_weight_sum_failure_stops_run():
+    """A marker pushed outside the box after step 100 fails the weight-sum
+    check (src/funlbm/lbm/lbm3d.py:84-86) and run() stops."""
+    lbm = make_case(particles=SPHERE)
+    lbm.config.max_step = 104
+    lbm.config.log_every = 1
+    lbm.init()
+    for s in range(1, 101):
+        lbm.step = s
+        lbm.run_step()
+    lbm.particle_swarm.lx[0] = torch.tensor([40.0, 40.0, 40.0], device=lbm.device)
+    lbm.step = 101
+    lbm.run_step()
+    lbm._log_step_info()
+    assert lbm.run_status is False
