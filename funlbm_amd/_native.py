@@ -79,6 +79,7 @@ SIGNATURES = {
     "funlbm_forcing": (c_int32, [LATP, P, P, P, P]),
     "funlbm_pack": (c_int32, [LATP, P, P, P]),
     "funlbm_equilibrium_state": (c_int32, [LATP, P, P, P, P]),
+    "funlbm_init_state": (c_int32, [LATP, P, P]),
     "funlbm_fused_step": (c_int32, [LATP, P, P, P, P, P, P, P, P, P]),
     "funlbm_materialize": (c_int32, [LATP, P, P, P, P, P, P, P, P, P, P, P]),
     "funlbm_copy_calibrate": (c_int32, [P, P, c_int64, P]),

@@ -49,7 +49,9 @@ class CaseSpec:
     tau: float = 0.6
     steps: int = 1000
     particles: List[ParticleSpec] = dataclasses.field(default_factory=list)
-    gravity: tuple = (0.0, 0.0, -2e-5)
+    # g of the buoyancy force (1 - rho_p/rho_f) V g: heavy particles move
+    # against g, so +z settles them towards z = 0
+    gravity: tuple = (0.0, 0.0, 2e-5)
     bcs: tuple = (nat.BC_WALL,) * 6
     uzero_mask: int = 0x3F
 
@@ -210,11 +212,11 @@ class BatchEngine:
         self.cases[cid * self.case_bytes : (cid + 1) * self.case_bytes] = torch.frombuffer(
             bytearray(desc), dtype=torch.uint8
         ).to(dev)
-        # state: the equilibrium at rho = 1, u = 0 (the reference's init and
-        # its two collisions before the first stream leave f = w)
+        # state: the reference's init (f = w, its moments, the two collisions
+        # before the first stream), as LBMD3 enters its fused steps
         esz = self.pool[0].element_size()
-        nat.call("funlbm_equilibrium_state", lat, None, None,
-                 ctypes.c_void_p(self.pool[0].data_ptr() + f_off * esz), nat.stream_handle(dev))
+        nat.call("funlbm_init_state", lat, ctypes.c_void_p(self.pool[0].data_ptr() + f_off * esz),
+                 nat.stream_handle(dev))
         for j in range(2):
             self.fol[j][3 * node_off : 3 * (node_off + n)].zero_()
         self.fol_tiles[tile_off : tile_off + n_tiles].zero_()

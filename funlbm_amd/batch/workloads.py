@@ -29,7 +29,8 @@ def settling_batch(n_cases=256, seed=0, steps=3000, side=(32, 96), qs=(15, 19), 
     """Independent settling cases: cubic-ish boxes with each side
     log-uniform in `side`, lattice by coin flip from `qs`, tau uniform, one
     sphere of uniform radius and density ratio started in the upper middle of
-    the box, walls on all faces, gravity along -z."""
+    the box, walls on all faces, settling towards z = 0 (g = +z in the
+    reference's buoyancy term (1 - rho_p/rho_f) V g)."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(n_cases):
@@ -43,7 +44,7 @@ def settling_batch(n_cases=256, seed=0, steps=3000, side=(32, 96), qs=(15, 19), 
         center = (center[0], center[1], min(center[2], nodes[2] - r - 4.0))
         out.append(CaseSpec(nodes=nodes, q=q, tau=t, steps=steps,
                             particles=[ParticleSpec(kind="sphere", r=r, rou=dens, center=center)],
-                            gravity=(0.0, 0.0, -gravity)))
+                            gravity=(0.0, 0.0, gravity)))
     return out
 
 
@@ -82,5 +83,5 @@ def serving_requests(rate, duration, seed=0, side=(32, 128), qs=(19, 27), n_part
                                           angle=_uniform_quat_rotvec(rng)))
         n_steps = int(round(math.exp(rng.uniform(math.log(steps[0]), math.log(steps[1])))))
         out.append((t, CaseSpec(nodes=nodes, q=q, tau=float(rng.uniform(*tau)), steps=n_steps,
-                                particles=parts, gravity=(0.0, 0.0, -gravity))))
+                                particles=parts, gravity=(0.0, 0.0, gravity))))
     return out

@@ -137,6 +137,34 @@ __global__ __launch_bounds__(kBlock) void k_equilibrium(KParams p, const float* 
   });
 }
 
+// The reference's initial state as the fused path enters it: f = rho w with
+// rho = 1, the moments of that f (FOL = the body force), then the extra
+// collision of LBMD3._init and the first step's cul_equ, both with those same
+// moments (src/funlbm/flow/d3/base.py:99-101, src/funlbm/lbm/lbm3d.py:29,
+// src/funlbm/lbm/base.py:161).  Computed in the storage's arithmetic type.
+template <class L, class S>
+__global__ __launch_bounds__(kBlock) void k_init_state(KParams p, typename S::elem* __restrict__ state) {
+  using T = typename S::T;
+  const int64_t node = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (node >= p.n) return;
+  const Node c = node_coords(p, node);
+  T f[L::Q];
+  static_for<0, L::Q>([&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    f[K] = (((T)1 + (T)0) + (T)0 - (T)0) * ((T)1 * wt<T>(L::W[K]));
+  });
+  const T F[3] = {(T)p.g[0], (T)p.g[1], (T)p.g[2]};
+  T rho, u[3];
+  moments<L, T>(p, c, f, F, rho, u);
+  collide<L, T>(f, u, rho, (T)p.tau, (T*)nullptr);
+  collide<L, T>(f, u, rho, (T)p.tau, (T*)nullptr);
+  static_for<0, L::Q>([&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
+    state[(int64_t)K * p.n + node] = S::store(f[K], wr, p.inv_scale, p.scale);
+  });
+}
+
 // ---------------------------------------------------------------------------
 // Fused pull step, one node per thread (body in fused_node.h).
 // ---------------------------------------------------------------------------
@@ -284,6 +312,23 @@ int funlbm_equilibrium_state(const funlbm_lattice* lat, const float* rho, const 
       hipLaunchKernelGGL((k_equilibrium<L, S>), dim3(grid_for(p.n)), dim3(kBlock), 0, s, p, rho, u,
                          (typename S::elem*)state);
       return check_launch("equilibrium");
+    });
+  });
+}
+
+int funlbm_init_state(const funlbm_lattice* lat, void* state, void* stream) {
+  KParams p;
+  int rc = make_params(lat, &p);
+  if (rc) return rc;
+  if (!state) return set_error(FUNLBM_EINVAL, "init_state: NULL state");
+  hipStream_t s = (hipStream_t)stream;
+  return dispatch_lattice(lat->q, [&](auto l) {
+    using L = decltype(l);
+    return dispatch_storage(lat->storage, [&](auto st) {
+      using S = decltype(st);
+      hipLaunchKernelGGL((k_init_state<L, S>), dim3(grid_for(p.n)), dim3(kBlock), 0, s, p,
+                         (typename S::elem*)state);
+      return check_launch("init_state");
     });
   });
 }

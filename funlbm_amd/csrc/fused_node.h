@@ -29,14 +29,15 @@ struct FusedIO {
   float* __restrict__ out_p;
 };
 
-template <class L, class S, bool FORCE, bool MAT>
-__device__ __forceinline__ void fused_node(const KParams& p, int64_t node, const FusedIO<S>& io,
-                                           bool new_here, bool old_here) {
+// The post-step f, u, rho of a node (the reference's tensors after the step),
+// without the collision.  Shared by fused_node and the observers below.
+template <class L, class S, bool FORCE>
+__device__ __forceinline__ void step_node(const KParams& p, int64_t node, const Node& c,
+                                          const FusedIO<S>& io, bool new_here, bool old_here,
+                                          typename S::T* f, typename S::T& rho,
+                                          typename S::T* u) {
   using T = typename S::T;
-  const Node c = node_coords(p, node);
-  T f[L::Q];
   pull_node<L, S>(p, io.src, c, node, io.rho_prev, f);
-
   T Fn[3] = {(T)0, (T)0, (T)0};
   if (FORCE) {
 #pragma unroll
@@ -53,8 +54,17 @@ __device__ __forceinline__ void fused_node(const KParams& p, int64_t node, const
       add_forcing<L, T>(f, u1, Fn, (T)p.tau);
     }
   }
-  T rho, u[3];
   moments<L, T>(p, c, f, Fn, rho, u);
+}
+
+template <class L, class S, bool FORCE, bool MAT>
+__device__ __forceinline__ void fused_node(const KParams& p, int64_t node, const FusedIO<S>& io,
+                                           bool new_here, bool old_here) {
+  using T = typename S::T;
+  const Node c = node_coords(p, node);
+  T f[L::Q];
+  T rho, u[3];
+  step_node<L, S, FORCE>(p, node, c, io, new_here, old_here, f, rho, u);
 
   if constexpr (MAT) {
     if (io.out_f) {

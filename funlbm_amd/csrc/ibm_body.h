@@ -21,6 +21,10 @@ constexpr float kPiF = 3.14159265358979323846f;
 
 // Stencil of a marker (src/funlbm/lbm/lbm3d.py:32-44): rl = max(floor(x) - 1, 0),
 // rr = min(rl + 4, size) with size = the config size (node count - 1).
+// rl is also capped at size before the integer conversion: a marker past the
+// upper face has an empty stencil either way (rr = size <= rl), and the cap
+// keeps lo + 3 far from integer overflow for any input, NaN and inf included
+// (fmaxf(NaN, 0) = 0).
 struct Stencil {
   int lo[3], hi[3];
 };
@@ -32,6 +36,7 @@ __device__ inline Stencil stencil_of(const KParams& p, const float x[3]) {
   for (int d = 0; d < 3; ++d) {
     float rl = floorf(x[d]) - 1.0f;
     rl = fmaxf(rl, 0.0f);
+    rl = fminf(rl, (float)size[d]);
     float rr = fminf(rl + 4.0f, (float)size[d]);
     s.lo[d] = (int)rl;
     s.hi[d] = (int)rr;
@@ -215,7 +220,8 @@ __device__ inline void wall_particle(const KParams& p, int b, int e, const float
     int ext[6];
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
-      ext[j] = si[j][0];
+      // every comparison fails on NaN coordinates; stay inside [b, e)
+      ext[j] = (si[j][0] >= b && si[j][0] < e) ? si[j][0] : b;
       const float c = lx[3 * ext[j] + (j % 3)];
       float d = 300.0f * (1.0f - fabsf(c - bounds[j]) / (2.0f * dx));
       d = fmaxf(d, 0.0f);

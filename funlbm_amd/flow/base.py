@@ -103,8 +103,11 @@ class FlowConfig(BaseConfig):
       uzero_faces  list of face names on which u is zeroed after each moment
                    update (default: all six, the reference's behaviour)
       body_force   [gx, gy, gz] added to FOL everywhere (default none)
-      gravity      [gx, gy, gz] gravity of the particles; the reference's is
-                   [gl, 0, 0] (src/funlbm/particle/base/base.py:138-142)
+      gravity      [gx, gy, gz], the vector g of the particles' buoyancy force
+                   (1 - rho_p/rho_f) V g; the reference's is [gl, 0, 0]
+                   (src/funlbm/particle/base/base.py:138-142).  With this sign a
+                   particle heavier than the fluid accelerates against g (the
+                   reference's free-fall example sets gl < 0).
     """
 
     def __init__(self, *args, **kwargs):

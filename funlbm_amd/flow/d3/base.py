@@ -205,10 +205,10 @@ class FlowD3(Flow):
         self._alloc_state()
         s = self._stream()
         if self._pristine and self.lat.storage != nat.STORE_F32:
-            # the equilibrium at rho = 1, u = 0 computed in the storage's type
-            # (collision leaves it unchanged)
+            # the initial state computed directly in the storage's arithmetic
+            # type (for float32 storage it equals the explicit sequence below)
             self.cul_equ()
-            nat.call("funlbm_equilibrium_state", self.lat, None, None, nat.ptr(self._state[self._cur]), s)
+            nat.call("funlbm_init_state", self.lat, nat.ptr(self._state[self._cur]), s)
         else:
             self.cul_equ()
             nat.call("funlbm_pack", self.lat, nat.ptr(self._f), nat.ptr(self._state[self._cur]), s)

@@ -113,6 +113,13 @@ int funlbm_pack(const funlbm_lattice* lat, const float* f, void* state, void* st
 int funlbm_equilibrium_state(const funlbm_lattice* lat, const float* rho, const float* u,
                              void* state, void* stream);
 
+/* The state the fused path starts from after the reference's init: f = w
+ * (rho = 1), the moments of it, then the extra collision of LBMD3._init and
+ * the first step's collision with those moments (src/funlbm/flow/d3/base.py:
+ * 99-101, src/funlbm/lbm/lbm3d.py:29, src/funlbm/lbm/base.py:161), computed in
+ * the storage's arithmetic type. */
+int funlbm_init_state(const funlbm_lattice* lat, void* state, void* stream);
+
 /* One fused pass replacing, for every node, the tail of LBMBase.run_step
  * (src/funlbm/lbm/base.py:159-175: f_stream, update_u_rou, cul_equ2,
  * update_u_rou) and the cul_equ that opens the next step (:161):

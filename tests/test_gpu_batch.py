@@ -17,7 +17,7 @@ def sphere_case(nodes, q=19, tau=0.6, r=4.0, rou=1.5, steps=30, center=None):
     center = center or tuple(n / 2 for n in nodes)
     return CaseSpec(nodes=nodes, q=q, tau=tau, steps=steps,
                     particles=[ParticleSpec(kind="sphere", r=r, rou=rou, center=center)],
-                    gravity=(0.0, 0.0, -2e-5))
+                    gravity=(0.0, 0.0, 2e-5))
 
 
 def lbmd3_for(spec, storage):
@@ -122,3 +122,23 @@ def test_retire_and_reuse_slot():
     fresh.step(40)
     assert torch.equal(eng.fields(again)[0], fresh.fields(f)[0])
     assert torch.equal(eng.particle_rows(again)[:, :15], fresh.particle_rows(f)[:, :15])
+
+
+def test_diverging_case_stays_in_bounds_and_isolated():
+    """Regression: a light small sphere whose explicit coupling diverges (its
+    markers leave the box and turn non-finite) keeps every index in range, and
+    the case next to it is untouched."""
+    unstable = CaseSpec(nodes=(28, 16, 30), q=19, tau=0.9, steps=140,
+                        particles=[ParticleSpec(kind="sphere", r=2.5, rou=1.2, center=(20, 8, 20))])
+    stable = sphere_case((20, 22, 26), steps=140)
+    eng = BatchEngine(storage="f32", max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8)
+    a = eng.add(unstable)
+    b = eng.add(stable)
+    eng.step(140)
+    torch.cuda.synchronize()
+    ref = BatchEngine(storage="f32", max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8)
+    c = ref.add(stable)
+    ref.step(140)
+    assert torch.equal(eng.fields(b)[0], ref.fields(c)[0])
+    rows = eng.particle_rows(a)[0, :3].cpu().numpy()
+    assert not np.all(np.abs(rows) < 1e3)  # it did diverge

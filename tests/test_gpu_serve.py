@@ -19,9 +19,9 @@ def small_requests():
         CaseSpec(nodes=(24, 18, 20), q=27, tau=0.7, steps=75,
                  particles=[ParticleSpec(kind="ellipsoid", abc=(2.5, 2.5, 4), rou=2.0, center=(12, 9, 11),
                                          angle=(0.2, 0.4, 0.1))]),
-        CaseSpec(nodes=(16, 16, 30), q=19, tau=0.9, steps=200,
+        CaseSpec(nodes=(28, 16, 30), q=19, tau=0.9, steps=200,
                  particles=[ParticleSpec(kind="sphere", r=3.0, rou=2.5, center=(8, 8, 20)),
-                            ParticleSpec(kind="sphere", r=2.5, rou=1.2, center=(8, 8, 9))]),
+                            ParticleSpec(kind="sphere", r=3.0, rou=2.0, center=(20, 8, 20))]),
         CaseSpec(nodes=(22, 22, 22), q=27, tau=0.55, steps=50,
                  particles=[ParticleSpec(kind="sphere", r=4.0, rou=3.0, center=(11, 11, 13))]),
         CaseSpec(nodes=(18, 26, 18), q=19, tau=0.8, steps=150,
