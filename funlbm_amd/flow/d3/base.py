@@ -369,12 +369,31 @@ class FlowD3(Flow):
             self._x = torch.stack([gx, gy, gz], dim=-1)
         return self._x
 
+    def stats(self):
+        """min, mean, max of the reference's f, u and rho as a double[9]
+        device tensor.  After fused steps it is one read of the state
+        (funlbm_flow_stats) instead of materialising the fields."""
+        if self._mode == "fused" and not self._mat_valid:
+            if getattr(self, "_stat_part", None) is None:
+                self._stat_part = torch.empty(9 * 2048, device=self.device, dtype=torch.float64)
+                self._stat_out = torch.empty(9, device=self.device, dtype=torch.float64)
+            a = 1 - self._cur
+            nat.call(
+                "funlbm_flow_stats", self.lat, nat.ptr(self._state[a]), self._fol_ptr(1 - self._fc),
+                self._fol_ptr(self._fc), nat.ptr(self._tiles[0]), nat.ptr(self._tiles[1]),
+                nat.ptr(self._rho_face[a]), nat.ptr(self._stat_part), 2048, nat.ptr(self._stat_out),
+                self._stream(),
+            )
+            return self._stat_out
+        vals = []
+        for t in (self.f, self.u, self.rou):
+            vals += [t.min().double(), t.double().mean(), t.max().double()]
+        return torch.stack(vals)
+
     def to_json(self):
         """min/mean/max of f, u, rho (src/funlbm/flow/base.py:183-206)."""
-        out = {}
-        for name, t in (("f", self.f), ("u", self.u), ("rho", self.rou)):
-            out[name] = [float(t.min()), float(t.mean()), float(t.max())]
-        return out
+        v = self.stats().tolist()
+        return {"f": v[0:3], "u": v[3:6], "rho": v[6:9]}
 
 
 def _lattice_class(name):

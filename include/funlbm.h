@@ -245,6 +245,18 @@ int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, in
                       const int32_t* traj_cap, int32_t grid_tiles, int32_t grid_markers,
                       int32_t grid_particles, int32_t with_ibm, void* stream);
 
+/* min, mean and max of the reference's f, u and rho after the last fused
+ * step, computed from the state without materialising the fields (the
+ * statistics of src/funlbm/flow/base.py:183-206 that the run loop logs every
+ * step, src/funlbm/lbm/base.py:105-144).  out: double[9] = f min, mean, max,
+ * u min, mean, max, rho min, mean, max.  partials: device scratch of
+ * 9 * n_partials doubles (one entry per block; n_partials >= 1 bounds the
+ * grid).  Deterministic: fixed-order reductions. */
+int funlbm_flow_stats(const funlbm_lattice* lat, const void* state_src, const float* fol_old,
+                      const float* fol_new, const uint32_t* fol_tiles, const int32_t* ctl,
+                      const float* rho_prev, double* partials, int32_t n_partials, double* out,
+                      void* stream);
+
 /* Plain device copy of n_bytes (16-byte aligned), used to calibrate the
  * measured copy bandwidth.  No reference counterpart. */
 int funlbm_copy_calibrate(const void* src, void* dst, int64_t n_bytes, void* stream);

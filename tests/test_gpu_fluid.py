@@ -97,11 +97,11 @@ def test_collide_moments_forcing(q):
     assert rel_linf(fl.f.cpu().numpy(), st.f) < 1e-6
 
 
-def run_solver(q, size, mu, bcs, steps, f0=None, fused=True, **flow_ext):
+def run_solver(q, size, mu, bcs, steps, f0=None, fused=True, storage="f32", **flow_ext):
     from funlbm_amd.lbm import LBMD3, Config
 
-    cfg = Config().from_json({"flow": flow_json(q, size, mu, bcs, **flow_ext), "log_every": 0,
-                              "fused": fused})
+    cfg = Config().from_json({"flow": flow_json(q, size, mu, bcs, storage=storage, **flow_ext),
+                              "log_every": 0, "fused": fused})
     lbm = LBMD3(config=cfg)
     lbm.init()
     if f0 is not None:
@@ -167,3 +167,21 @@ def test_poiseuille_config1_1000_steps():
     ef = rel_linf(lbm.flow.f.cpu().numpy(), st.f)
     eu = rel_linf(lbm.flow.u.cpu().numpy(), st.u)
     assert ef < 1e-5 and eu < 1e-4, (ef, eu)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_flow_stats_from_state(storage):
+    """min/mean/max read straight from the fused state equal those of the
+    materialised fields (the run loop's per-step statistics)."""
+    size = (9, 11, 13)
+    bcs = BC_SETS["channel"]
+    nodes = tuple(s + 1 for s in size)
+    f0 = random_state(19, nodes, seed=2)
+    lbm = run_solver(19, size, 0.05, bcs, 7, f0=f0, storage=storage, body_force=[0.0, 1e-5, 0.0])
+    fast = lbm.flow.stats().cpu().numpy()
+    want = []
+    for t in (lbm.flow.f, lbm.flow.u, lbm.flow.rou):
+        want += [t.min().item(), t.double().mean().item(), t.max().item()]
+    want = np.array(want)
+    assert np.array_equal(fast[[0, 2, 3, 5, 6, 8]], want[[0, 2, 3, 5, 6, 8]])
+    assert np.allclose(fast[[1, 4, 7]], want[[1, 4, 7]], rtol=1e-9, atol=1e-15)
