@@ -319,6 +319,38 @@ class BatchEngine:
         for _ in range(n):
             self._launch()
 
+    def autotune(self, steps: int = 10, candidates=None):
+        """Chooses the persistent grid of the fused multi-lattice kernel by
+        timing the first steps of the real run with each candidate (results do
+        not depend on the grid: every kernel is a grid-stride loop whose
+        per-node and per-marker work is independent of it).  Returns the
+        timings; the winner stays in self.grid."""
+        self._publish()
+        dev = self.device
+        cus = torch.cuda.get_device_properties(dev).multi_processor_count
+        n_tiles = int(self.hdr[H_TILES].item())
+        n_mark = max(1, int(self.hdr[H_MARKERS].item()))
+        self.grid["markers"] = max(1, min(-(-n_mark // 4), 8 * cus))
+        self.grid["particles"] = max(1, min(int(self.hdr[H_PARTICLES].item()), 4 * cus))
+        if candidates is None:
+            candidates = sorted({c for c in (2 * cus, 4 * cus, 8 * cus, 16 * cus, n_tiles) if 0 < c <= max(n_tiles, 1)})
+        timings = {}
+        for c in candidates:
+            self.grid["tiles"] = int(c)
+            self._launch()  # settle
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            for _ in range(steps):
+                self._launch()
+            b.record()
+            torch.cuda.synchronize(dev)
+            timings[int(c)] = a.elapsed_time(b) / steps
+        best = min(timings, key=timings.get)
+        self.grid["tiles"] = best
+        return {"grid_tiles": best, "ms_per_step": {str(k): round(v, 4) for k, v in timings.items()},
+                "steps_used": (steps + 1) * len(candidates)}
+
     def capture(self, n: int):
         """Captures n steps in a HIP graph (replayed by run_graph)."""
         self._publish()
