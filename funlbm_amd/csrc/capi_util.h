@@ -46,6 +46,22 @@ inline int storage_elem_bytes(int storage) {
   }
 }
 
+// Multiplier and shift with floor(n / d) = (n * m) >> s for all n < 2^31
+// (round-up method of Granlund and Montgomery: s = 31 + ceil(log2 d),
+// m = floor(2^s / d) + 1 < 2^32).
+inline bool magic_div(int d, uint32_t* m, int* s) {
+  if (d < 1) return false;
+  int l = 0;
+  while ((1LL << l) < d) ++l;
+  const int sh = 31 + l;
+  const unsigned __int128 two = (unsigned __int128)1 << sh;
+  const unsigned __int128 mm = two / (unsigned __int128)d + 1;
+  if (mm >= ((unsigned __int128)1 << 32)) return false;
+  *m = (uint32_t)mm;
+  *s = sh;
+  return true;
+}
+
 // Validates a lattice descriptor and fills the kernel parameters.
 inline int make_params(const funlbm_lattice* lat, KParams* p) {
   if (!lat) return set_error(FUNLBM_EINVAL, "lattice descriptor is NULL");
@@ -79,6 +95,15 @@ inline int make_params(const funlbm_lattice* lat, KParams* p) {
   }
   p->uzero = lat->uzero_mask & 0x3f;
   p->tau = lat->tau;
+  // the reference's float32 arithmetic on the host: correctly rounded
+  p->rtau = 1.0f / lat->tau;
+  p->omega = 1.0f - 1.0f / (2.0f * lat->tau);
+  p->tau_d = (double)lat->tau;
+  p->rtau_d = 1.0 / p->tau_d;
+  p->omega_d = 1.0 - 1.0 / (2.0 * p->tau_d);
+  if (!magic_div(p->nz, &p->mz, &p->sz) || !magic_div(p->ny, &p->my, &p->sy))
+    return set_error(FUNLBM_EINVAL, "grid extent without a 32-bit division multiplier");
+  p->off32 = ((uint64_t)p->n * (uint64_t)storage_elem_bytes(lat->storage)) <= 0xffffffffULL;
   p->rho0 = lat->rho0;
   p->scale = dev ? lat->dev_scale : 1.0f;
   p->inv_scale = dev ? 1.0f / lat->dev_scale : 1.0f;

@@ -24,7 +24,7 @@ __global__ __launch_bounds__(kBlock) void k_collide(KParams p, float* __restrict
 #pragma unroll
   for (int k = 0; k < L::Q; ++k) fl[k] = f[(int64_t)k * p.n + node];
   const float uu[3] = {u[node], u[p.n + node], u[2 * p.n + node]};
-  collide<L, float>(fl, uu, rho[node], p.tau, e);
+  collide<L, float>(fl, uu, rho[node], relax_of<float>(p), e);
 #pragma unroll
   for (int k = 0; k < L::Q; ++k) {
     f[(int64_t)k * p.n + node] = fl[k];
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(kBlock) void k_forcing(KParams p, float* __restrict
 #pragma unroll
   for (int k = 0; k < L::Q; ++k) fl[k] = f[(int64_t)k * p.n + node];
   const float uu[3] = {u[node], u[p.n + node], u[2 * p.n + node]};
-  add_forcing<L, float>(fl, uu, F, p.tau);
+  add_forcing<L, float>(fl, uu, F, relax_of<float>(p));
 #pragma unroll
   for (int k = 0; k < L::Q; ++k) f[(int64_t)k * p.n + node] = fl[k];
 }
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(kBlock) void k_equilibrium(KParams p, const float* 
   T f[L::Q], e[L::Q];
 #pragma unroll
   for (int k = 0; k < L::Q; ++k) f[k] = (T)0;
-  collide<L, T>(f, uu, r, (T)1, e);
+  collide<L, T>(f, uu, r, Relax<T>{(T)1, (T)1, (T)0}, e);
   static_for<0, L::Q>([&](auto kc) {
     constexpr int K = decltype(kc)::value;
     const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
@@ -156,8 +156,8 @@ __global__ __launch_bounds__(kBlock) void k_init_state(KParams p, typename S::el
   const T F[3] = {(T)p.g[0], (T)p.g[1], (T)p.g[2]};
   T rho, u[3];
   moments<L, T>(p, c, f, F, rho, u);
-  collide<L, T>(f, u, rho, (T)p.tau, (T*)nullptr);
-  collide<L, T>(f, u, rho, (T)p.tau, (T*)nullptr);
+  collide<L, T>(f, u, rho, relax_of<T>(p), (T*)nullptr);
+  collide<L, T>(f, u, rho, relax_of<T>(p), (T*)nullptr);
   static_for<0, L::Q>([&](auto kc) {
     constexpr int K = decltype(kc)::value;
     const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
@@ -188,10 +188,19 @@ __global__ __launch_bounds__(kBlock) void k_fused(
   fused_node<L, S, FORCE, MAT>(p, node, io, new_here, old_here);
 }
 
+// Plain copy with four independent 16-byte loads in flight per thread.
 __global__ __launch_bounds__(kBlock) void k_copy(const uint4* __restrict__ src,
                                                  uint4* __restrict__ dst, int64_t n16) {
   const int64_t stride = (int64_t)gridDim.x * kBlock;
-  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+  int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  for (; i + 3 * stride < n16; i += 4 * stride) {
+    const uint4 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+    dst[i] = a;
+    dst[i + stride] = b;
+    dst[i + 2 * stride] = c;
+    dst[i + 3 * stride] = d;
+  }
+  for (; i < n16; i += stride) dst[i] = src[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -383,7 +392,7 @@ int funlbm_copy_calibrate(const void* src, void* dst, int64_t n_bytes, void* str
   if (!src || !dst || n_bytes <= 0 || (n_bytes % 16) != 0)
     return set_error(FUNLBM_EINVAL, "copy: bad arguments");
   const int64_t n16 = n_bytes / 16;
-  int blocks = (int)std::min<int64_t>((n16 + kBlock - 1) / kBlock, 256 * 64);
+  int blocks = (int)std::min<int64_t>((n16 + 4 * kBlock - 1) / (4 * kBlock), 256 * 32);
   hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(kBlock), 0, (hipStream_t)stream,
                      (const uint4*)src, (uint4*)dst, n16);
   return check_launch("copy");

@@ -51,7 +51,7 @@ __device__ __forceinline__ void step_node(const KParams& p, int64_t node, const 
         Fo[d] = (T)(old_here ? io.fol_old[(int64_t)d * p.n + node] + p.g[d] : p.g[d]);
       T r1, u1[3];
       moments<L, T>(p, c, f, Fo, r1, u1);
-      add_forcing<L, T>(f, u1, Fn, (T)p.tau);
+      add_forcing<L, T>(f, u1, Fn, relax_of<T>(p));
     }
   }
   moments<L, T>(p, c, f, Fn, rho, u);
@@ -78,7 +78,7 @@ __device__ __forceinline__ void fused_node(const KParams& p, int64_t node, const
     if (io.out_rho) io.out_rho[node] = (float)rho;
     if (io.out_p) io.out_p[node] = (float)(rho / (T)3);
   } else {
-    collide<L, T>(f, u, rho, (T)p.tau, (T*)nullptr);
+    collide<L, T>(f, u, rho, relax_of<T>(p), (T*)nullptr);
     if (p.has_moving && io.rho_next) {
       bool mf = false;
 #pragma unroll
@@ -86,11 +86,22 @@ __device__ __forceinline__ void fused_node(const KParams& p, int64_t node, const
         mf |= (p.bc[face] == FUNLBM_BC_WALL_WITH_SPEED) && on_face(p, c, face);
       if (mf) io.rho_next[node] = (float)rho;
     }
-    static_for<0, L::Q>([&](auto kc) {
-      constexpr int K = decltype(kc)::value;
-      const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
-      io.dst[(int64_t)K * p.n + node] = S::store(f[K], wr, p.inv_scale, p.scale);
-    });
+    using E = typename S::elem;
+    if (p.off32) {
+      const uint32_t off = (uint32_t)node * (uint32_t)sizeof(E);
+      static_for<0, L::Q>([&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
+        char* base = (char*)io.dst + (int64_t)K * p.n * (int64_t)sizeof(E);
+        *(E*)(base + off) = S::store(f[K], wr, p.inv_scale, p.scale);
+      });
+    } else {
+      static_for<0, L::Q>([&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
+        io.dst[(int64_t)K * p.n + node] = S::store(f[K], wr, p.inv_scale, p.scale);
+      });
+    }
   }
 }
 

@@ -141,15 +141,55 @@ struct KParams {
   int nx, ny, nz;
   int sxy;    // ny * nz, the x stride of the node index
   int64_t n;  // nodes (< 2^31, checked on the host)
+  // node -> (x, y, z) by multiply-shift: q = (n * m) >> s is floor(n / d) for
+  // every n < 2^31 (Granlund-Montgomery; m = floor(2^s / d) + 1, s = 31 + ceil(log2 d))
+  uint32_t mz, my;
+  int sz, sy;
+  int off32;  // n * sizeof(element) < 2^32: 32-bit byte offsets are safe
   int bc[6];
   int uzero;
   int has_moving;  // any WALL_WITH_SPEED face
   int has_body;    // body_force != 0
-  float tau;
+  float tau;       // float32 tau as the reference's field
+  float rtau;      // float32(1 / tau)
+  float omega;     // 1 - 1/(2 tau) in float32, the forcing prefactor
+  double tau_d, rtau_d, omega_d;  // the same in double for float64 storage
   float rho0, scale, inv_scale;
   float g[3];
   float uw[6][3];
 };
+
+// Relaxation constants in the arithmetic type T.
+template <class T>
+struct Relax {
+  T tau, rtau, omega;
+};
+template <class T>
+__device__ inline Relax<T> relax_of(const KParams& p);
+template <>
+__device__ inline Relax<float> relax_of<float>(const KParams& p) {
+  return {p.tau, p.rtau, p.omega};
+}
+template <>
+__device__ inline Relax<double> relax_of<double>(const KParams& p) {
+  return {p.tau_d, p.rtau_d, p.omega_d};
+}
+
+// x / c correctly rounded in three operations, given rc = RN(1/c): q = x rc,
+// the residual x - q c is exact with an FMA, and q + residual * rc rounds to
+// RN(x / c) (Markstein's theorem; binary floating point, round to nearest,
+// results away from the subnormal range).  Used wherever the reference
+// divides, so quotients stay bit-identical to a true division.
+__device__ __forceinline__ float divc(float x, float c, float rc) {
+  const float q = x * rc;
+  const float e = __builtin_fmaf(-q, c, x);
+  return __builtin_fmaf(e, rc, q);
+}
+__device__ __forceinline__ double divc(double x, double c, double rc) {
+  const double q = x * rc;
+  const double e = __builtin_fma(-q, c, x);
+  return __builtin_fma(e, rc, q);
+}
 
 // Constants of the reference: cs = float32(sqrt(1/3)), cs^2 and cs^4 rounded
 // to float32 as torch computes `cs**2` and `cs2**2` on float32 scalars
@@ -158,13 +198,17 @@ template <class T>
 struct Cs;
 template <>
 struct Cs<float> {
-  __device__ static inline float cs2() { return __uint_as_float(0x3eaaaaaau); }  // fl32(cs*cs)
-  __device__ static inline float cs4() { return cs2() * cs2(); }
+  __device__ static inline float cs2() { return __uint_as_float(0x3eaaaaaau); }   // fl32(cs*cs)
+  __device__ static inline float cs4() { return __uint_as_float(0x3de38e37u); }   // fl32(cs2*cs2)
+  __device__ static inline float rcs2() { return __uint_as_float(0x40400001u); }  // RN(1/cs2)
+  __device__ static inline float rcs4() { return __uint_as_float(0x41100001u); }  // RN(1/cs4)
 };
 template <>
 struct Cs<double> {
   __device__ static inline double cs2() { return 1.0 / 3.0; }
   __device__ static inline double cs4() { return 1.0 / 9.0; }
+  __device__ static inline double rcs2() { return 3.0; }  // RN(1/(1/3)) in double
+  __device__ static inline double rcs4() { return 9.0; }  // RN(1/(1/9)) in double
 };
 
 template <class T>
@@ -176,12 +220,16 @@ __device__ inline T wt(double w) {
 struct Node {
   int x, y, z;
 };
+__device__ inline uint32_t fastdiv(uint32_t n, uint32_t m, int s) {
+  return (uint32_t)(((uint64_t)n * m) >> s);
+}
+
 __device__ inline Node node_coords(const KParams& p, int64_t node) {
   Node c;
   const uint32_t nd = (uint32_t)node;
-  const uint32_t t = nd / (uint32_t)p.nz;
+  const uint32_t t = fastdiv(nd, p.mz, p.sz);
   c.z = (int)(nd - t * (uint32_t)p.nz);
-  const uint32_t x = t / (uint32_t)p.ny;
+  const uint32_t x = fastdiv(t, p.my, p.sy);
   c.y = (int)(t - x * (uint32_t)p.ny);
   c.x = (int)x;
   return c;
@@ -284,15 +332,30 @@ __device__ inline void pull_node(const KParams& p, const typename S::elem* __res
                                  const Node& c, int64_t node, const float* __restrict__ rho_prev,
                                  typename S::T* f) {
   using T = typename S::T;
+  using E = typename S::elem;
   if (interior_node(p, c)) {
-    const int nd = (int)node;
-    static_for<0, L::Q>([&](auto kc) {
-      constexpr int K = decltype(kc)::value;
-      constexpr int ex = L::E[K][0], ey = L::E[K][1], ez = L::E[K][2];
-      const int sn = nd - (ex * p.sxy + ey * p.nz + ez);
-      f[K] = S::load(src[(int64_t)K * p.n + sn], (float)(wt<float>(L::W[K]) * p.rho0), p.inv_scale,
-                     p.scale);
-    });
+    if (p.off32) {
+      // uniform base of plane K shifted by -e_k, plus one shared 32-bit byte
+      // offset: a scalar base + vector offset load per direction
+      const uint32_t off = (uint32_t)node * (uint32_t)sizeof(E);
+      static_for<0, L::Q>([&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        constexpr int ex = L::E[K][0], ey = L::E[K][1], ez = L::E[K][2];
+        const char* base = (const char*)src +
+                           ((int64_t)K * p.n - (int64_t)(ex * p.sxy + ey * p.nz + ez)) * (int64_t)sizeof(E);
+        f[K] = S::load(*(const E*)(base + off), (float)(wt<float>(L::W[K]) * p.rho0), p.inv_scale,
+                       p.scale);
+      });
+    } else {
+      const int nd = (int)node;
+      static_for<0, L::Q>([&](auto kc) {
+        constexpr int K = decltype(kc)::value;
+        constexpr int ex = L::E[K][0], ey = L::E[K][1], ez = L::E[K][2];
+        const int sn = nd - (ex * p.sxy + ey * p.nz + ez);
+        f[K] = S::load(src[(int64_t)K * p.n + sn], (float)(wt<float>(L::W[K]) * p.rho0), p.inv_scale,
+                       p.scale);
+      });
+    }
     return;
   }
   static_for<0, L::Q>([&](auto kc) {
@@ -326,9 +389,10 @@ __device__ inline void moments(const KParams& p, const Node& c, const T* f, cons
     if (L::E[k][2] == -1) jz = jz - f[k];
   }
   rho = r;
-  u[0] = (jx + F[0] / (T)2) / r;
-  u[1] = (jy + F[1] / (T)2) / r;
-  u[2] = (jz + F[2] / (T)2) / r;
+  const T rr = (T)1 / r;
+  u[0] = divc(jx + F[0] / (T)2, r, rr);
+  u[1] = divc(jy + F[1] / (T)2, r, rr);
+  u[2] = divc(jz + F[2] / (T)2, r, rr);
   if (uzero_at(p, c)) {
     u[0] = (T)0;
     u[1] = (T)0;
@@ -365,16 +429,26 @@ __device__ inline double sqrt_t<double>(double v) {
 // (1 + tmp + tmp^2/2 - u2) * (rho w_k), tmp = (u.e_k)/cs^2,
 // u2 = |u|^2/cs^2/2 with |u| = sqrt(u.u) squared again as torch does.
 // f_k <- f_k - (f_k - feq_k)/tau.  If feq != nullptr it receives feq_k.
+// e_rev(k) . u is exactly -(e_k . u), so tmp of a reverse direction is the
+// negated tmp of its partner.
 template <class L, class T>
-__device__ inline void collide(T* f, const T u[3], T rho, T tau, T* feq) {
+__device__ inline void collide(T* f, const T u[3], T rho, const Relax<T>& rx, T* feq) {
   const T nrm = sqrt_t<T>(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
-  const T u2 = nrm * nrm / Cs<T>::cs2() / (T)2;
+  const T u2 = divc(nrm * nrm, Cs<T>::cs2(), Cs<T>::rcs2()) / (T)2;
+  T tmp[L::Q];
   static_for<0, L::Q>([&](auto kc) {
     constexpr int K = decltype(kc)::value;
-    const T tmp = edotu<L, K, T>(u) / Cs<T>::cs2();
-    const T e = ((((T)1 + tmp) + tmp * tmp / (T)2) - u2) * (rho * wt<T>(L::W[K]));
+    if constexpr (L::REV[K] < K)
+      tmp[K] = -tmp[L::REV[K]];
+    else
+      tmp[K] = divc(edotu<L, K, T>(u), Cs<T>::cs2(), Cs<T>::rcs2());
+  });
+  static_for<0, L::Q>([&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    const T t = tmp[K];
+    const T e = ((((T)1 + t) + t * t / (T)2) - u2) * (rho * wt<T>(L::W[K]));
     if (feq) feq[K] = e;
-    f[K] = f[K] - (f[K] - e) / tau;
+    f[K] = f[K] - divc(f[K] - e, rx.tau, rx.rtau);
   });
 }
 
@@ -382,16 +456,16 @@ __device__ inline void collide(T* f, const T u[3], T rho, T tau, T* feq) {
 // t1 = (e_a - u)/cs^2, t2 = (u e_a) e_a / cs^4 (elementwise),
 // t3 = sum_d (t1 + t2)_d F_d, f_a += ((1 - 1/(2 tau)) w_a) t3.
 template <class L, class T>
-__device__ inline void add_forcing(T* f, const T u[3], const T F[3], T tau) {
-  const T omega = (T)1 - (T)1 / ((T)2 * tau);
+__device__ inline void add_forcing(T* f, const T u[3], const T F[3], const Relax<T>& rx) {
+  const T omega = rx.omega;
   static_for<0, L::Q>([&](auto kc) {
     constexpr int K = decltype(kc)::value;
     T t3 = (T)0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const T e = (T)L::E[K][d];
-      const T t1 = (e - u[d]) / Cs<T>::cs2();
-      const T t2 = ((u[d] * e) * e) / Cs<T>::cs4();
+      const T t1 = divc(e - u[d], Cs<T>::cs2(), Cs<T>::rcs2());
+      const T t2 = divc((u[d] * e) * e, Cs<T>::cs4(), Cs<T>::rcs4());
       t3 = (d == 0) ? (t1 + t2) * F[d] : t3 + (t1 + t2) * F[d];
     }
     f[K] = f[K] + (omega * wt<T>(L::W[K])) * t3;
