@@ -78,6 +78,35 @@ def time_steps(lbm, steps, warmup):
     return ms / steps, wall
 
 
+def time_fused_kernel(lbm, reps=50):
+    """Average duration (ms) of one funlbm_fused_step launch: `reps` launches
+    back to back on the stream the step runs on, ping-ponging the flow's two
+    state buffers, bracketed by events recorded on that same stream."""
+    import torch
+
+    from funlbm_amd import _native as nat
+
+    fl = lbm.flow
+    fl.begin_step()
+    a, b = fl._state[fl._cur], fl._state[1 - fl._cur]
+    stream = torch.cuda.current_stream()
+    s = nat.stream_handle()
+
+    def launch(src, dst):
+        nat.call("funlbm_fused_step", fl.lat, nat.ptr(src), nat.ptr(dst), None, None, None, None, None, None, s)
+
+    for i in range(4):
+        launch(a, b) if i % 2 == 0 else launch(b, a)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for i in range(reps):
+        launch(a, b) if i % 2 == 0 else launch(b, a)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
 def copy_bandwidth(nbytes=2 << 30, reps=10):
     """Measured copy peak: read + write bytes of a plain copy kernel / time."""
     import torch
@@ -227,6 +256,28 @@ def bench_serve(rate=20.0, duration=10.0, seed=0, steps=(1000, 20000), budget=20
                       max_particles=4 * len(reqs) + 4, max_cases=len(reqs) + 1, max_q=27, traj_cap=tick)
     srv = Server(eng, hbm_budget=budget, tick_steps=tick)
     stats = srv.run([Request(rid=i, spec=s, arrival=t) for i, (t, s) in enumerate(reqs)])
+    del srv, eng
+    torch.cuda.empty_cache()
+    # error of the served storage against float64 on a sample request solved alone
+    spec = reqs[0][1]
+    fields = {}
+    for st in ("f64", storage):
+        e = BatchEngine(storage=st, max_nodes=spec.n + 256, max_markers=sum(len(markers_for(p)[0]) for p in spec.particles) + 8,
+                        max_particles=len(spec.particles) + 1, max_cases=1, max_q=27, traj_cap=8)
+        cid = e.add(spec)
+        e.step(spec.steps)
+        u, _ = e.fields(cid)
+        fields[st] = (u.double().cpu().numpy(), e.particle_rows(cid)[:, 0:3].cpu().numpy())
+        del e
+        torch.cuda.empty_cache()
+    u64, c64 = fields["f64"]
+    us, cs = fields[storage]
+    scale = float(np.abs(u64).max())
+    stats["error_check"] = {
+        "sample": f"request 0 alone: {spec.nodes[0]}^3 D3Q{spec.q}, {len(spec.particles)} particles, {spec.steps} steps",
+        "u_rel_linf_vs_f64": float(np.abs(us - u64).max() / scale) if scale > 0 else None,
+        "centroid_abs_vs_f64": float(np.abs(cs - c64).max()) if len(c64) else None,
+    }
     stats.update({
         "workload": f"Poisson {rate} req/s for {duration} s, sides 32-128, D3Q19/27, 1-4 particles, "
                     f"steps {steps[0]}-{steps[1]}, {storage}-deviation storage, {budget / 1e9:.0f} GB budget",
@@ -235,8 +286,6 @@ def bench_serve(rate=20.0, duration=10.0, seed=0, steps=(1000, 20000), budget=20
     for k in ("wall_s", "busy_s", "solves_per_s", "mlups", "p50_latency_s", "p99_latency_s"):
         if stats.get(k) is not None:
             stats[k] = round(stats[k], 3)
-    del srv, eng
-    torch.cuda.empty_cache()
     return stats
 
 
@@ -275,7 +324,10 @@ def main():
     mlups = n / (ms * 1e-3) / 1e6
     elem = {"f32": 4, "f16": 2, "bf16": 2, "f64": 8}[args.storage]
     bytes_per_lu = 2 * args.q * elem
-    achieved = mlups * 1e6 * bytes_per_lu / 1e9
+    # roofline of the dominant kernel: algorithmic bytes of one launch (read
+    # and write every population of every node once) / its average duration
+    kernel_ms = time_fused_kernel(lbm)
+    achieved = n * bytes_per_lu / (kernel_ms * 1e-3) / 1e9
     out = {
         "metric": METRIC,
         "value": round(mlups, 1),
@@ -304,6 +356,8 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
             "bytes_per_lu": bytes_per_lu,
+            "bytes_per_launch": n * bytes_per_lu,
+            "kernel_ms": round(kernel_ms, 4),
             "kernel": "k_fused (funlbm_fused_step)",
         },
     }
@@ -312,6 +366,7 @@ def main():
     if not args.no_f16 and args.storage == "f32":
         lbm16 = build_solver(args.size, args.q, "f16")
         ms16, _ = time_steps(lbm16, args.steps, args.warmup)
+        k16 = time_fused_kernel(lbm16)
         del lbm16
         torch.cuda.empty_cache()
         m16 = n / (ms16 * 1e-3) / 1e6
@@ -319,7 +374,8 @@ def main():
             "mlups": round(m16, 1),
             "ms_per_step": round(ms16, 4),
             "speedup_vs_f32": round(m16 / mlups, 3),
-            "achieved_gbs": round(m16 * 1e6 * 2 * args.q * 2 / 1e9, 1),
+            "kernel_ms": round(k16, 4),
+            "achieved_gbs": round(n * 2 * args.q * 2 / (k16 * 1e-3) / 1e9, 1),
         }
     if not args.no_batch:
         try:

@@ -156,5 +156,14 @@ def stream_handle(device=None):
     return c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
+def state_plane(lat: Lattice) -> int:
+    """Elements per direction plane of a fused-state buffer of `lat`
+    (funlbm_state_bytes / (q * element size)); at least the node count."""
+    nbytes = c_int64()
+    check(lib().funlbm_state_bytes(ctypes.byref(lat), ctypes.byref(nbytes)), "funlbm_state_bytes")
+    esz = torch.tensor([], dtype=STORAGE_DTYPES[lat.storage]).element_size()
+    return int(nbytes.value) // (int(lat.q) * esz)
+
+
 def call(name: str, *args) -> None:
     check(getattr(lib(), name)(*args), name)

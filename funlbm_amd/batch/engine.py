@@ -112,6 +112,7 @@ class _Slot:
     m_count: int
     p_off: int
     p_count: int
+    plane: int
 
 
 class BatchEngine:
@@ -179,17 +180,22 @@ class BatchEngine:
         m = sum(len(markers_for(p)[0]) for p in spec.particles)
         return (
             bool(self.free_cases)
-            and self.elems.fits(spec.q * spec.n)
+            and self.elems.fits(spec.q * self._plane(spec))
             and self.nodes.fits(spec.n)
             and self.tiles.fits(-(-spec.n // TILE))
             and (m == 0 or self.markers.fits(m))
             and (not spec.particles or self.parts.fits(len(spec.particles)))
         )
 
+    def _plane(self, spec: CaseSpec) -> int:
+        return nat.state_plane(lattice_for(spec, self.storage, self.dev_scale))
+
     @staticmethod
     def case_bytes_needed(spec: CaseSpec, storage_bytes: int = 2) -> int:
         """HBM held by a case: two state blocks, two FOL blocks, owner words."""
-        return spec.n * (2 * spec.q * storage_bytes + 2 * 12 + 8)
+        storage = {2: nat.STORE_F16, 4: nat.STORE_F32, 8: nat.STORE_F64}[storage_bytes]
+        plane = nat.state_plane(lattice_for(spec, storage, 1024.0))
+        return 2 * spec.q * plane * storage_bytes + spec.n * (2 * 12 + 8)
 
     def add(self, spec: CaseSpec) -> int:
         """Admits a case; returns its slot id.  Raises MemoryError if it does not fit."""
@@ -197,7 +203,8 @@ class BatchEngine:
             raise MemoryError("case does not fit the pools")
         dev = self.device
         n = spec.n
-        f_off = self.elems.alloc(spec.q * n)
+        plane = self._plane(spec)
+        f_off = self.elems.alloc(spec.q * plane)
         node_off = self.nodes.alloc(n)
         n_tiles = -(-n // TILE)
         tile_off = self.tiles.alloc(n_tiles)
@@ -261,7 +268,7 @@ class BatchEngine:
         self.origin_ok[cid] = dyn[4]
         self.case_marker_begin[cid] = dyn[5]
         self.slots[cid] = _Slot(spec, lat, f_off, node_off, tile_off, n_tiles, m_off, m_count, p_off,
-                                len(spec.particles))
+                                len(spec.particles), plane)
         self._dirty = True
         return cid
 
@@ -274,7 +281,7 @@ class BatchEngine:
         if s.p_count:
             self.particle_case[s.p_off : s.p_off + s.p_count] = -1
             self.parts.free(s.p_off, s.p_count)
-        self.elems.free(s.f_off, s.spec.q * s.spec.n)
+        self.elems.free(s.f_off, s.spec.q * s.plane)
         self.nodes.free(s.node_off, s.spec.n)
         self.tiles.free(s.tile_off, s.n_tiles)
         self.free_cases.append(cid)

@@ -86,20 +86,23 @@ __device__ __forceinline__ void fused_node(const KParams& p, int64_t node, const
         mf |= (p.bc[face] == FUNLBM_BC_WALL_WITH_SPEED) && on_face(p, c, face);
       if (mf) io.rho_next[node] = (float)rho;
     }
-    using E = typename S::elem;
+    // The state is written once and read back only by the next step, after
+    // every other plane has streamed through the caches: non-temporal stores.
+    using R = typename S::raw;
     if (p.off32) {
-      const uint32_t off = (uint32_t)node * (uint32_t)sizeof(E);
+      const uint32_t off = (uint32_t)node * (uint32_t)sizeof(R);
       static_for<0, L::Q>([&](auto kc) {
         constexpr int K = decltype(kc)::value;
         const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
-        char* base = (char*)io.dst + (int64_t)K * p.n * (int64_t)sizeof(E);
-        *(E*)(base + off) = S::store(f[K], wr, p.inv_scale, p.scale);
+        char* base = (char*)io.dst + (int64_t)K * p.ps * (int64_t)sizeof(R);
+        __builtin_nontemporal_store(S::store_raw(f[K], wr, p.inv_scale, p.scale), (R*)(base + off));
       });
     } else {
+      R* dst = (R*)io.dst;
       static_for<0, L::Q>([&](auto kc) {
         constexpr int K = decltype(kc)::value;
         const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
-        io.dst[(int64_t)K * p.n + node] = S::store(f[K], wr, p.inv_scale, p.scale);
+        __builtin_nontemporal_store(S::store_raw(f[K], wr, p.inv_scale, p.scale), dst + (int64_t)K * p.ps + node);
       });
     }
   }

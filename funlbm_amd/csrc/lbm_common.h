@@ -98,16 +98,22 @@ struct StoreF32 {
   using T = float;
   using elem = float;
   static constexpr int code = FUNLBM_STORE_F32;
+  using raw = float;
   __device__ static inline float load(elem v, float, float, float) { return v; }
   __device__ static inline elem store(float f, float, float, float) { return f; }
+  __device__ static inline float load_raw(raw v, float, float, float) { return v; }
+  __device__ static inline raw store_raw(float f, float, float, float) { return f; }
 };
 
 struct StoreF64 {
   using T = double;
   using elem = double;
   static constexpr int code = FUNLBM_STORE_F64;
+  using raw = double;
   __device__ static inline double load(elem v, double, double, double) { return v; }
   __device__ static inline elem store(double f, double, double, double) { return f; }
+  __device__ static inline double load_raw(raw v, double, double, double) { return v; }
+  __device__ static inline raw store_raw(double f, double, double, double) { return f; }
 };
 
 struct StoreF16 {
@@ -119,6 +125,14 @@ struct StoreF16 {
   }
   __device__ static inline elem store(float f, float wrho0, float, float scale) {
     return __float2half_rn((f - wrho0) * scale);
+  }
+  // the same conversions on the 16-bit patterns (vector loads and stores)
+  using raw = unsigned short;
+  __device__ static inline float load_raw(raw v, float wrho0, float inv_scale, float s) {
+    return load(__ushort_as_half(v), wrho0, inv_scale, s);
+  }
+  __device__ static inline raw store_raw(float f, float wrho0, float is, float scale) {
+    return __half_as_ushort(store(f, wrho0, is, scale));
   }
 };
 
@@ -132,6 +146,13 @@ struct StoreBF16 {
   __device__ static inline elem store(float f, float wrho0, float, float scale) {
     return __float2bfloat16(((f - wrho0) * scale));
   }
+  using raw = unsigned short;
+  __device__ static inline float load_raw(raw v, float wrho0, float inv_scale, float s) {
+    return load(__builtin_bit_cast(elem, v), wrho0, inv_scale, s);
+  }
+  __device__ static inline raw store_raw(float f, float wrho0, float is, float scale) {
+    return __builtin_bit_cast(raw, store(f, wrho0, is, scale));
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -141,6 +162,8 @@ struct KParams {
   int nx, ny, nz;
   int sxy;    // ny * nz, the x stride of the node index
   int64_t n;  // nodes (< 2^31, checked on the host)
+  int64_t ps; // plane stride of the fused state in elements: n, padded on large
+              // grids so that the Q planes do not start at power-of-two distances
   // node -> (x, y, z) by multiply-shift: q = (n * m) >> s is floor(n / d) for
   // every n < 2^31 (Granlund-Montgomery; m = floor(2^s / d) + 1, s = 31 + ceil(log2 d))
   uint32_t mz, my;
@@ -342,7 +365,7 @@ __device__ inline void pull_node(const KParams& p, const typename S::elem* __res
         constexpr int K = decltype(kc)::value;
         constexpr int ex = L::E[K][0], ey = L::E[K][1], ez = L::E[K][2];
         const char* base = (const char*)src +
-                           ((int64_t)K * p.n - (int64_t)(ex * p.sxy + ey * p.nz + ez)) * (int64_t)sizeof(E);
+                           ((int64_t)K * p.ps - (int64_t)(ex * p.sxy + ey * p.nz + ez)) * (int64_t)sizeof(E);
         f[K] = S::load(*(const E*)(base + off), (float)(wt<float>(L::W[K]) * p.rho0), p.inv_scale,
                        p.scale);
       });
@@ -352,7 +375,7 @@ __device__ inline void pull_node(const KParams& p, const typename S::elem* __res
         constexpr int K = decltype(kc)::value;
         constexpr int ex = L::E[K][0], ey = L::E[K][1], ez = L::E[K][2];
         const int sn = nd - (ex * p.sxy + ey * p.nz + ez);
-        f[K] = S::load(src[(int64_t)K * p.n + sn], (float)(wt<float>(L::W[K]) * p.rho0), p.inv_scale,
+        f[K] = S::load(src[(int64_t)K * p.ps + sn], (float)(wt<float>(L::W[K]) * p.rho0), p.inv_scale,
                        p.scale);
       });
     }
@@ -361,7 +384,7 @@ __device__ inline void pull_node(const KParams& p, const typename S::elem* __res
   static_for<0, L::Q>([&](auto kc) {
     constexpr int K = decltype(kc)::value;
     const Src s = source_of<L, K>(p, c);
-    const int64_t idx = (int64_t)s.k * p.n + lin(p, s.x, s.y, s.z);
+    const int64_t idx = (int64_t)s.k * p.ps + lin(p, s.x, s.y, s.z);
     // The element was stored with the weight of s.k; rev(k) and k share a
     // weight on every lattice.
     T v = S::load(src[idx], (float)(wt<float>(L::W[K]) * p.rho0), p.inv_scale, p.scale);
@@ -416,13 +439,17 @@ __device__ inline T edotu(const T u[3]) {
 
 template <class T>
 __device__ inline T sqrt_t(T v);
+// Correctly rounded square roots, as numpy and torch compute them.  The
+// compiler's lowering of the sqrt builtin refines the hardware's faithful
+// v_sqrt_f32 with the residual test of its two neighbours; ROCm's __fsqrt_rn
+// is the bare faithful instruction and differs in the last bit now and then.
 template <>
 __device__ inline float sqrt_t<float>(float v) {
-  return __fsqrt_rn(v);
+  return __builtin_sqrtf(v);
 }
 template <>
 __device__ inline double sqrt_t<double>(double v) {
-  return __dsqrt_rn(v);
+  return __builtin_sqrt(v);
 }
 
 // BGK collision (src/funlbm/flow/d3/base.py:150-159).  feq_k =

@@ -190,7 +190,8 @@ class FlowD3(Flow):
         dt = nat.STORAGE_DTYPES[self.lat.storage]
         for i in range(2):
             if self._state[i] is None:
-                self._state[i] = torch.empty((self.q, self.n), device=self.device, dtype=dt)
+                plane = nat.state_plane(self.lat)
+                self._state[i] = torch.empty((self.q, plane), device=self.device, dtype=dt)
         if self.has_moving_wall:
             for i in range(2):
                 if self._rho_face[i] is None:

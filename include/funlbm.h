@@ -71,7 +71,10 @@ int funlbm_abi_version(void);
 /* Thread-local description of the last error.  No reference counterpart. */
 const char* funlbm_last_error(void);
 
-/* Bytes of one population buffer (q * nodes * element size) for `lat`. */
+/* Bytes of one fused-state buffer for `lat`: q planes of P elements of
+ * lat->storage, P = nodes on grids under 65536 nodes, else nodes rounded up to
+ * 256 plus 4096 (so that the planes do not start at power-of-two distances).
+ * Plane k of the state starts at element k * P.  No reference counterpart. */
 int funlbm_state_bytes(const funlbm_lattice* lat, int64_t* bytes);
 
 /* ---- reference-structured operations (explicit path, float, in place) ---- */
@@ -107,9 +110,11 @@ int funlbm_forcing(const funlbm_lattice* lat, float* f, const float* u, const fl
  * buffer of lat->storage.  No reference counterpart (storage format). */
 int funlbm_pack(const funlbm_lattice* lat, const float* f, void* state, void* stream);
 
-/* State buffer = equilibrium populations feq(rho, u) computed in the storage's
- * arithmetic type (rho NULL = 1, u NULL = 0); FlowD3.init's f = rho w is the
- * u = 0 case (src/funlbm/flow/d3/base.py:94-100). */
+/* out = equilibrium populations feq(rho, u), [q][x][y][z] in lat->storage
+ * (planes of n elements, the layout funlbm_pack reads, not the padded state
+ * layout), computed in the storage's arithmetic type (rho NULL = 1, u NULL =
+ * 0); FlowD3.init's f = rho w is the u = 0 case
+ * (src/funlbm/flow/d3/base.py:94-100). */
 int funlbm_equilibrium_state(const funlbm_lattice* lat, const float* rho, const float* u,
                              void* state, void* stream);
 

@@ -36,3 +36,23 @@ def random_state(q, nodes, seed, amp=1e-2):
     f = orc.equilibrium(lat, rho, u)
     f = f + (amp * 0.1 * rng.standard_normal(f.shape) * lat.w).astype(np.float32)
     return f.astype(np.float32)
+
+
+def deviation_round(f, w, storage, scale=1024.0, rho0=1.0):
+    """The state storage's round trip of float32 populations f[..., q]: f16
+    and bf16 keep (f - w rho0) * scale rounded to nearest-even, and read back
+    as v / scale + w rho0, all in float32 (the kernels' StoreF16/StoreBF16)."""
+    f = np.asarray(f, dtype=np.float32)
+    wr = (np.asarray(w, dtype=np.float32) * np.float32(rho0)).astype(np.float32)
+    s = np.float32(scale)
+    inv = np.float32(1.0) / s
+    d = ((f - wr) * s).astype(np.float32)
+    if storage == "f16":
+        r = d.astype(np.float16).astype(np.float32)
+    elif storage == "bf16":
+        u = d.view(np.uint32).astype(np.uint64)
+        u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+        r = u.astype(np.uint32).view(np.float32)
+    else:
+        raise ValueError(storage)
+    return (r * inv + wr).astype(np.float32)

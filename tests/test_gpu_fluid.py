@@ -9,7 +9,7 @@ import pytest
 import torch
 
 from oracle import lbm_oracle as orc
-from tests.helpers import flow_json, random_state, rel_linf
+from tests.helpers import deviation_round, flow_json, random_state, rel_linf
 
 pytestmark = pytest.mark.gpu
 
@@ -187,18 +187,55 @@ def test_flow_stats_from_state(storage):
     assert np.allclose(fast[[1, 4, 7]], want[[1, 4, 7]], rtol=1e-9, atol=1e-15)
 
 
+@pytest.mark.parametrize("size", [(11, 9, 13), (9, 10, 15)])
 @pytest.mark.parametrize("q", [15, 19, 27])
-def test_fused_bitwise_equal_to_oracle(q):
+def test_fused_bitwise_equal_to_oracle(q, size):
     """Every quotient of the kernels is the correctly rounded one (three-
     operation division by a rounded reciprocal) and every sum runs in the
     oracle's order, so the float32 state is bit-identical to the oracle's
-    after many steps, body force and walls included."""
-    size = (11, 9, 13)
+    after many steps, body force and walls included.  With 16 nodes along z
+    the interior runs through the four-nodes-per-thread vector path."""
     bcs = BC_SETS["channel"]
     nodes = tuple(s + 1 for s in size)
     f0 = random_state(q, nodes, seed=4)
     g = [1e-5, 0.0, 2e-5]
     lbm = run_solver(q, size, 0.07, bcs, 40, f0=f0, body_force=g)
     st = run_oracle(q, size, 0.07, bcs, 40, f0=f0, body_force=g)
+    got = lbm.flow.f.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), st.f.view(np.uint32)), rel_linf(got, st.f)
+
+
+def run_oracle_rounded(q, size, mu, bcs, steps, f0, storage, body_force=None, scale=1024.0):
+    """The oracle's steps with the state storage's rounding applied to the
+    post-collision populations, where the fused step stores them."""
+    nodes = tuple(int(s) + 1 for s in size)
+    st = orc.init_flow(q, nodes, mu=mu, bc=bcs, body_force=body_force)
+    orc.init_solver(st)
+    st.f = f0.copy()
+    orc.update_u_rou(st)
+    for _ in range(steps):
+        orc.cul_equ(st)
+        st.f = deviation_round(st.f, st.lat.w, storage, scale)
+        orc.f_stream(st)
+        orc.update_u_rou(st)
+        st.FOL = np.zeros_like(st.FOL)
+        orc.cul_equ2(st)
+        orc.update_u_rou(st)
+    return st
+
+
+@pytest.mark.parametrize("size", [(9, 10, 15), (11, 9, 13)])
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("q", [19, 27])
+def test_deviation_storage_bitwise_equal_to_rounded_oracle(q, storage, size):
+    """f16/bf16 state: the fused path equals the oracle with the storage's
+    round trip applied after every collision, bit for bit (16 nodes along z
+    exercise the vector path, 14 the per-node path)."""
+    bcs = BC_SETS["channel"]
+    nodes = tuple(s + 1 for s in size)
+    f0 = random_state(q, nodes, seed=6)
+    g = [2e-5, 0.0, -1e-5]
+    lbm = run_solver(q, size, 0.07, bcs, 30, f0=f0, storage=storage, body_force=g)
+    st = run_oracle_rounded(q, size, 0.07, bcs, 30, f0, storage, body_force=g)
     got = lbm.flow.f.cpu().numpy()
     assert np.array_equal(got.view(np.uint32), st.f.view(np.uint32)), rel_linf(got, st.f)
