@@ -62,13 +62,16 @@ inline bool magic_div(int d, uint32_t* m, int* s) {
   return true;
 }
 
-// Plane stride of the fused state.  Planes of a large grid (from 2^16 nodes)
-// are rounded up to 256 elements and then staggered by 4096 more, so that the
-// 2Q streams of the fused step do not all start at the same offset modulo a
-// large power of two (a 256^3 grid has 2^24-element planes); measured +3..7%
-// on the MI355X for the 19-plane pull-and-store pattern.
+// Plane stride of the fused state: at least n + 1, so that the one-element
+// overreach of the corner directions' shifted loads on a z-face node of the
+// last plane stays inside the buffer (the loaded value is then replaced by the
+// face rule).  Planes of a large grid (from 2^16 nodes) are rounded up to 256
+// elements and then staggered by 4096 more, so that the 2Q streams of the
+// fused step do not all start at the same offset modulo a large power of two
+// (a 256^3 grid has 2^24-element planes); measured +3..7% on the MI355X for
+// the 19-plane pull-and-store pattern.  Smaller grids round up to 64 plus 64.
 inline int64_t state_plane_stride(int64_t n) {
-  if (n < (1 << 16)) return n;
+  if (n < (1 << 16)) return ((n + 63) / 64) * 64 + 64;
   return ((n + 255) / 256) * 256 + 4096;
 }
 

@@ -188,7 +188,6 @@ __global__ __launch_bounds__(kBlock) void k_fused(
   fused_node<L, S, FORCE, MAT>(p, node, io, new_here, old_here);
 }
 
-// Plain copy with four independent 16-byte loads in flight per thread.
 // Plain copy, one 16-byte element per thread (the fastest plain form measured
 // on the MI355X: about 6.1 TB/s read + write on 2 GiB).
 __global__ __launch_bounds__(kBlock) void k_copy(const uint4* __restrict__ src,

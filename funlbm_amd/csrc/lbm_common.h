@@ -347,16 +347,46 @@ __device__ inline T moving_wall_term(const KParams& p, int face, T rho) {
   return tmp * eu / Cs<T>::cs2();
 }
 
+// The z-face override of a node whose x and y are inner (only the bottom or
+// top rule of source_of can apply there): directions with e_z = DZ take, on a
+// periodic face, plane K at the same (x, y) on the opposite z face; on a wall,
+// plane rev(K) at the node itself, less the moving-wall term.
+template <class L, class S, int DZ>
+__device__ inline void zface_pull(const KParams& p, const typename S::elem* __restrict__ src, int64_t node,
+                                  const float* __restrict__ rho_prev, typename S::T* f) {
+  using T = typename S::T;
+  constexpr int face = DZ > 0 ? 4 : 5;
+  const int bc = p.bc[face];
+  static_for<0, L::Q>([&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    if constexpr (L::E[K][2] == DZ) {
+      const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
+      if (bc == FUNLBM_BC_PERIODICAL) {
+        const int64_t idx = (int64_t)K * p.ps + node + (DZ > 0 ? (p.nz - 1) : -(p.nz - 1));
+        f[K] = S::load(src[idx], wr, p.inv_scale, p.scale);
+      } else {
+        T v = S::load(src[(int64_t)L::REV[K] * p.ps + node], wr, p.inv_scale, p.scale);
+        if (bc == FUNLBM_BC_WALL_WITH_SPEED) v = v - moving_wall_term<L, K, T>(p, face, (T)rho_prev[node]);
+        f[K] = v;
+      }
+    }
+  });
+}
+
 // Pull all Q post-stream populations of node c out of a state buffer.
-// Interior nodes take the plain shifted source node - e_k; face nodes go
-// through the full override logic of source_of.
+// Nodes with inner x and y load every plane at the shifted source node - e_k
+// (in bounds even on a z face, where the row wraps into the neighbouring
+// row), then a z-face node replaces its e_z-pointing directions by the face
+// rule; nodes on an x or y face go through the full override logic of
+// source_of.  In node order a wave covers part of one z-row, so the x/y-face
+// rows are whole waves and only the z-face lanes take the short fix-up.
 template <class L, class S>
 __device__ inline void pull_node(const KParams& p, const typename S::elem* __restrict__ src,
                                  const Node& c, int64_t node, const float* __restrict__ rho_prev,
                                  typename S::T* f) {
   using T = typename S::T;
   using E = typename S::elem;
-  if (interior_node(p, c)) {
+  if ((c.x > 0) & (c.x < p.nx - 1) & (c.y > 0) & (c.y < p.ny - 1)) {
     if (p.off32) {
       // uniform base of plane K shifted by -e_k, plus one shared 32-bit byte
       // offset: a scalar base + vector offset load per direction
@@ -379,6 +409,8 @@ __device__ inline void pull_node(const KParams& p, const typename S::elem* __res
                        p.scale);
       });
     }
+    if (c.z == 0) zface_pull<L, S, 1>(p, src, node, rho_prev, f);
+    if (c.z == p.nz - 1) zface_pull<L, S, -1>(p, src, node, rho_prev, f);
     return;
   }
   static_for<0, L::Q>([&](auto kc) {

@@ -50,12 +50,22 @@ def compare(cfg_json: dict, steps: int, storages=("f32", "f16", "bf16")) -> Dict
     return out
 
 
-def poiseuille(size=(31, 31, 63), mu=0.1, g=1e-6) -> dict:
-    """Plain Poiseuille channel: walls at x, periodic y and z, body force g_z."""
+def poiseuille(size=(31, 31, 63), mu=0.1, g=1e-5) -> dict:
+    """Poiseuille channel: walls at x, periodic y and z, body force g_z, u
+    zeroed on the wall faces only (the reference zeroes it on all six faces,
+    which on the periodic faces acts as a momentum sink)."""
     b = {f: {"code": "PERIODICAL"} for f in FACES}
     b["input"] = b["output"] = {"code": "WALL"}
     return {"flow": {"param_type": "D3Q19", "size": list(size), "param": {"mu": mu}, "boundary": b,
-                     "body_force": [0.0, 0.0, g]}, "log_every": 0}
+                     "body_force": [0.0, 0.0, g], "uzero_faces": ["input", "output"]}, "log_every": 0}
+
+
+def cavity(size=(47, 47, 47), mu=0.05, lid=0.05) -> dict:
+    """Lid-driven cavity: walls on every face, the top one moving along x."""
+    b = {f: {"code": "WALL"} for f in FACES}
+    b["top"] = {"code": "WALL_WITH_SPEED", "uw": [lid, 0.0, 0.0]}
+    return {"flow": {"param_type": "D3Q19", "size": list(size), "param": {"mu": mu}, "boundary": b},
+            "log_every": 0}
 
 
 def settling_sphere(size=(63, 63, 127), mu=1 / 30, r=6, center=(32, 32, 100), rou=1.5, g=2e-5) -> dict:

@@ -72,9 +72,10 @@ int funlbm_abi_version(void);
 const char* funlbm_last_error(void);
 
 /* Bytes of one fused-state buffer for `lat`: q planes of P elements of
- * lat->storage, P = nodes on grids under 65536 nodes, else nodes rounded up to
- * 256 plus 4096 (so that the planes do not start at power-of-two distances).
- * Plane k of the state starts at element k * P.  No reference counterpart. */
+ * lat->storage.  P = nodes rounded up to 64 plus 64 on grids under 65536
+ * nodes, else nodes rounded up to 256 plus 4096 (so that the planes do not
+ * start at power-of-two distances); P > nodes always.  Plane k of the state
+ * starts at element k * P.  No reference counterpart. */
 int funlbm_state_bytes(const funlbm_lattice* lat, int64_t* bytes);
 
 /* ---- reference-structured operations (explicit path, float, in place) ---- */

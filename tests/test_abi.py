@@ -62,4 +62,26 @@ def test_argument_errors_need_no_device():
         lat.bc[i] = nat.BC_WALL
     nbytes = ctypes.c_int64()
     assert nat.lib().funlbm_state_bytes(ctypes.byref(lat), ctypes.byref(nbytes)) == 0
-    assert nbytes.value == 19 * 64 * 4
+    # planes of 64 nodes rounded up to 64, plus 64 elements of slack
+    assert nbytes.value == 19 * 128 * 4
+
+
+@pytest.mark.parametrize("dims,storage", [((4, 4, 4), 0), ((7, 9, 11), 1), ((64, 64, 64), 0), ((256, 256, 256), 1),
+                                          ((255, 255, 255), 0)])
+def test_state_plane_has_slack_and_is_staggered(dims, storage):
+    """Every state plane holds more than the node count (the z-face corner
+    loads overreach by one element); large planes are not a power of two."""
+    lat = nat.Lattice()
+    lat.q = 27
+    lat.nx, lat.ny, lat.nz = dims
+    lat.tau = 0.8
+    lat.storage = storage
+    lat.dev_scale = 1024.0
+    lat.rho0 = 1.0
+    for i in range(6):
+        lat.bc[i] = nat.BC_PERIODICAL
+    n = dims[0] * dims[1] * dims[2]
+    plane = nat.state_plane(lat)
+    assert plane > n
+    if n >= 1 << 16:
+        assert plane % 256 == 0 and plane % 65536 != 0
