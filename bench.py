@@ -107,6 +107,21 @@ def time_fused_kernel(lbm, reps=50):
     return e0.elapsed_time(e1) / reps
 
 
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_fused_d3q19_256_traffic.md")
+
+
+def profiled_traffic(storage, size, q):
+    """HBM bytes per k_fused launch from the committed rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes (copy-calibrated) of this workload, or None."""
+    if size != 256 or q != 19 or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == storage:
+            return int(parts[2])
+    return None
+
+
 def copy_bandwidth(nbytes=2 << 30, reps=10):
     """Measured copy peak: read + write bytes of a plain copy kernel / time."""
     import torch
@@ -354,7 +369,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic(args.storage, args.size, args.q),
+            "traffic_source": "profiles/k_fused_d3q19_256_traffic.md (rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1)",
             "bytes_per_lu": bytes_per_lu,
             "bytes_per_launch": n * bytes_per_lu,
             "kernel_ms": round(kernel_ms, 4),
