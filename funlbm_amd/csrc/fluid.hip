@@ -106,7 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_pack(KParams p, const float* __restr
   static_for<0, L::Q>([&](auto kc) {
     constexpr int K = decltype(kc)::value;
     const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
-    state[(int64_t)K * p.ps + node] =
+    state[state_index<L::Q>(p, K, node)] =
         S::store((typename S::T)f[(int64_t)K * p.n + node], wr, p.inv_scale, p.scale);
   });
 }
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(kBlock) void k_init_state(KParams p, typename S::el
   static_for<0, L::Q>([&](auto kc) {
     constexpr int K = decltype(kc)::value;
     const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
-    state[(int64_t)K * p.ps + node] = S::store(f[K], wr, p.inv_scale, p.scale);
+    state[state_index<L::Q>(p, K, node)] = S::store(f[K], wr, p.inv_scale, p.scale);
   });
 }
 
@@ -216,7 +216,7 @@ int funlbm_state_bytes(const funlbm_lattice* lat, int64_t* bytes) {
   int rc = make_params(lat, &p);
   if (rc) return rc;
   if (!bytes) return set_error(FUNLBM_EINVAL, "bytes is NULL");
-  *bytes = (int64_t)lat->q * p.ps * (int64_t)storage_elem_bytes(lat->storage);
+  *bytes = state_elems(lat->q, p.n, p.ts) * (int64_t)storage_elem_bytes(lat->storage);
   return 0;
 }
 

@@ -88,23 +88,21 @@ __device__ __forceinline__ void fused_node(const KParams& p, int64_t node, const
     }
     // The state is written once and read back only by the next step, after
     // every other plane has streamed through the caches: non-temporal stores.
+    // The wave's 64 nodes lie in one state tile (every caller runs waves of 64
+    // consecutive nodes from a multiple of 64): uniform tile base + in-tile
+    // byte offset, as in pull_node.
     using R = typename S::raw;
-    if (p.off32) {
-      const uint32_t off = (uint32_t)node * (uint32_t)sizeof(R);
-      static_for<0, L::Q>([&](auto kc) {
-        constexpr int K = decltype(kc)::value;
-        const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
-        char* base = (char*)io.dst + (int64_t)K * p.ps * (int64_t)sizeof(R);
-        __builtin_nontemporal_store(S::store_raw(f[K], wr, p.inv_scale, p.scale), (R*)(base + off));
-      });
-    } else {
-      R* dst = (R*)io.dst;
-      static_for<0, L::Q>([&](auto kc) {
-        constexpr int K = decltype(kc)::value;
-        const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
-        __builtin_nontemporal_store(S::store_raw(f[K], wr, p.inv_scale, p.scale), dst + (int64_t)K * p.ps + node);
-      });
-    }
+    constexpr int Q = L::Q;
+    const int ts = p.ts;
+    const uint32_t tmask = (1u << ts) - 1u;
+    const int tn = (__builtin_amdgcn_readfirstlane((int)node) & ~63) >> ts;
+    const uint32_t lb = ((uint32_t)node & tmask) * (uint32_t)sizeof(R);
+    static_for<0, Q>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
+      char* base = (char*)io.dst + ((int64_t)Q + (int64_t)tn * Q + K) * ((int64_t)1 << ts) * (int64_t)sizeof(R);
+      __builtin_nontemporal_store(S::store_raw(f[K], wr, p.inv_scale, p.scale), (R*)(base + lb));
+    });
   }
 }
 

@@ -120,7 +120,7 @@ __device__ inline void interp_marker(const KParams& p, int m, int lane,
     if constexpr (FROM_STATE) {
       const Node c{lw.x, lw.y, lw.z};
       T f[L::Q];
-      pull_node<L, S>(p, state, c, node, rho_face, f);
+      pull_node<L, S, false>(p, state, c, node, rho_face, f);
       T F[3];
 #pragma unroll
       for (int d = 0; d < 3; ++d)

@@ -162,13 +162,20 @@ struct KParams {
   int nx, ny, nz;
   int sxy;    // ny * nz, the x stride of the node index
   int64_t n;  // nodes (< 2^31, checked on the host)
-  int64_t ps; // plane stride of the fused state in elements: n, padded on large
-              // grids so that the Q planes do not start at power-of-two distances
+  int ts;     // tile shift of the fused-state layout (state_index)
+  // Per direction k, the pull source node - s_k (s_k = e_k . (sxy, nz, 1))
+  // split as s_k = q_k T + r_k, 0 <= r_k < T: soff[k] = byte offset of element
+  // (k, tile -q_k, in-tile -r_k) from the state's tile 0 minus the slack, i.e.
+  // ((Q - q_k Q + k) T - r_k) * element size; srk[k] = r_k.
+  int64_t soff[27];
+  uint32_t srk[27];
+  // bit w: the sources of some direction straddle a tile boundary for the
+  // wave at in-tile start 64 w (some r_k in (64 w, 64 w + 64))
+  uint64_t mixmask;
   // node -> (x, y, z) by multiply-shift: q = (n * m) >> s is floor(n / d) for
   // every n < 2^31 (Granlund-Montgomery; m = floor(2^s / d) + 1, s = 31 + ceil(log2 d))
   uint32_t mz, my;
   int sz, sy;
-  int off32;  // n * sizeof(element) < 2^32: 32-bit byte offsets are safe
   int bc[6];
   int uzero;
   int has_moving;  // any WALL_WITH_SPEED face
@@ -334,6 +341,19 @@ __device__ inline int64_t lin(const KParams& p, int x, int y, int z) {
   return (int64_t)(x * p.sxy + y * p.nz + z);
 }
 
+// Fused-state layout.  Nodes are grouped into tiles of T = 2^ts consecutive
+// nodes; a tile holds its nodes' Q direction planes one after another (T
+// elements each).  One slack tile precedes tile 0 and one follows the last, so
+// the one-node overreach of a corner direction's shifted load on a z-face node
+// (node -1 or node n, whose value the face rule then replaces) stays inside the
+// buffer.  Element (k, j) for j in [-1, n]:
+//   Q T + ((j >> ts) Q + k) T + (j & (T - 1)).
+template <int Q>
+__device__ inline int64_t state_index(const KParams& p, int k, int64_t j) {
+  const int64_t tile = j >> p.ts;  // floor (arithmetic shift), -1 for j = -1
+  return ((int64_t)Q + tile * Q + k) * ((int64_t)1 << p.ts) + (j & (((int64_t)1 << p.ts) - 1));
+}
+
 // Moving-wall correction 2 (rho w_k) (e_k . uw) / cs^2 in the reference's
 // operation order (src/funlbm/flow/d3/base.py:204-213).
 template <class L, int K, class T>
@@ -362,10 +382,10 @@ __device__ inline void zface_pull(const KParams& p, const typename S::elem* __re
     if constexpr (L::E[K][2] == DZ) {
       const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
       if (bc == FUNLBM_BC_PERIODICAL) {
-        const int64_t idx = (int64_t)K * p.ps + node + (DZ > 0 ? (p.nz - 1) : -(p.nz - 1));
+        const int64_t idx = state_index<L::Q>(p, K, node + (DZ > 0 ? (p.nz - 1) : -(p.nz - 1)));
         f[K] = S::load(src[idx], wr, p.inv_scale, p.scale);
       } else {
-        T v = S::load(src[(int64_t)L::REV[K] * p.ps + node], wr, p.inv_scale, p.scale);
+        T v = S::load(src[state_index<L::Q>(p, L::REV[K], node)], wr, p.inv_scale, p.scale);
         if (bc == FUNLBM_BC_WALL_WITH_SPEED) v = v - moving_wall_term<L, K, T>(p, face, (T)rho_prev[node]);
         f[K] = v;
       }
@@ -380,32 +400,57 @@ __device__ inline void zface_pull(const KParams& p, const typename S::elem* __re
 // rule; nodes on an x or y face go through the full override logic of
 // source_of.  In node order a wave covers part of one z-row, so the x/y-face
 // rows are whole waves and only the z-face lanes take the short fix-up.
-template <class L, class S>
+//
+// ALIGNED: the 64 lanes of the wave hold 64 consecutive nodes starting at a
+// multiple of 64 (every full-lattice kernel).  The wave then lies in one
+// state tile, and the source of direction K, node - s_K, lies in one tile or
+// straddles two: the tile and in-tile start are wave-uniform scalars and a
+// lane adds only its in-tile byte offset, as in a plain copy.
+template <class L, class S, bool ALIGNED = true>
 __device__ inline void pull_node(const KParams& p, const typename S::elem* __restrict__ src,
                                  const Node& c, int64_t node, const float* __restrict__ rho_prev,
                                  typename S::T* f) {
   using T = typename S::T;
   using E = typename S::elem;
+  constexpr int Q = L::Q;
   if ((c.x > 0) & (c.x < p.nx - 1) & (c.y > 0) & (c.y < p.ny - 1)) {
-    if (p.off32) {
-      // uniform base of plane K shifted by -e_k, plus one shared 32-bit byte
-      // offset: a scalar base + vector offset load per direction
-      const uint32_t off = (uint32_t)node * (uint32_t)sizeof(E);
-      static_for<0, L::Q>([&](auto kc) {
-        constexpr int K = decltype(kc)::value;
-        constexpr int ex = L::E[K][0], ey = L::E[K][1], ez = L::E[K][2];
-        const char* base = (const char*)src +
-                           ((int64_t)K * p.ps - (int64_t)(ex * p.sxy + ey * p.nz + ez)) * (int64_t)sizeof(E);
-        f[K] = S::load(*(const E*)(base + off), (float)(wt<float>(L::W[K]) * p.rho0), p.inv_scale,
-                       p.scale);
-      });
+    if constexpr (ALIGNED) {
+      const int ts = p.ts;
+      const uint32_t tmask = (1u << ts) - 1u;
+      const int nw = __builtin_amdgcn_readfirstlane((int)node) & ~63;  // the wave's first node
+      const uint32_t l0 = (uint32_t)nw & tmask;                        // its in-tile start
+      const uint32_t local = (uint32_t)node & tmask;
+      const uint32_t lb = local * (uint32_t)sizeof(E);
+      // the wave's tile, and one tile of Q planes less one plane (tile t,
+      // plane K -> tile t - 1, plane K)
+      const char* tb = (const char*)src + (int64_t)(nw >> ts) * Q * ((int64_t)sizeof(E) << ts);
+      const uint32_t back = (uint32_t)((Q - 1) * ((int64_t)sizeof(E) << ts));
+      if (!((p.mixmask >> (l0 >> 6)) & 1)) {
+        // every direction's sources lie in one tile: scalar base + the lane's
+        // in-tile offset, tile tn - q_K (rk <= l0) or tn - q_K - 1
+        static_for<0, Q>([&](auto kc) {
+          constexpr int K = decltype(kc)::value;
+          const char* ua = tb + p.soff[K];  // element (K, tile tn - q_K, in-tile -r_K)
+          const char* sb = p.srk[K] <= l0 ? ua : ua - back;
+          f[K] = S::load(*(const E*)(sb + lb), (float)(wt<float>(L::W[K]) * p.rho0), p.inv_scale, p.scale);
+        });
+      } else {
+        // a wave at a tile edge: per lane, the source tile of a straddling direction
+        static_for<0, Q>([&](auto kc) {
+          constexpr int K = decltype(kc)::value;
+          const uint32_t rk = p.srk[K];
+          const char* ua = tb + p.soff[K];
+          const char* sb = rk <= l0 ? ua : ua - back;
+          const uint32_t vo = (rk > l0) & (rk < l0 + 64) ? lb + (local >= rk ? back : 0u) : lb;
+          f[K] = S::load(*(const E*)(sb + vo), (float)(wt<float>(L::W[K]) * p.rho0), p.inv_scale, p.scale);
+        });
+      }
     } else {
-      const int nd = (int)node;
-      static_for<0, L::Q>([&](auto kc) {
+      static_for<0, Q>([&](auto kc) {
         constexpr int K = decltype(kc)::value;
         constexpr int ex = L::E[K][0], ey = L::E[K][1], ez = L::E[K][2];
-        const int sn = nd - (ex * p.sxy + ey * p.nz + ez);
-        f[K] = S::load(src[(int64_t)K * p.ps + sn], (float)(wt<float>(L::W[K]) * p.rho0), p.inv_scale,
+        const int64_t sn = node - (ex * p.sxy + ey * p.nz + ez);
+        f[K] = S::load(src[state_index<Q>(p, K, sn)], (float)(wt<float>(L::W[K]) * p.rho0), p.inv_scale,
                        p.scale);
       });
     }
@@ -413,10 +458,10 @@ __device__ inline void pull_node(const KParams& p, const typename S::elem* __res
     if (c.z == p.nz - 1) zface_pull<L, S, -1>(p, src, node, rho_prev, f);
     return;
   }
-  static_for<0, L::Q>([&](auto kc) {
+  static_for<0, Q>([&](auto kc) {
     constexpr int K = decltype(kc)::value;
     const Src s = source_of<L, K>(p, c);
-    const int64_t idx = (int64_t)s.k * p.ps + lin(p, s.x, s.y, s.z);
+    const int64_t idx = state_index<Q>(p, s.k, lin(p, s.x, s.y, s.z));
     // The element was stored with the weight of s.k; rev(k) and k share a
     // weight on every lattice.
     T v = S::load(src[idx], (float)(wt<float>(L::W[K]) * p.rho0), p.inv_scale, p.scale);

@@ -62,15 +62,16 @@ def test_argument_errors_need_no_device():
         lat.bc[i] = nat.BC_WALL
     nbytes = ctypes.c_int64()
     assert nat.lib().funlbm_state_bytes(ctypes.byref(lat), ctypes.byref(nbytes)) == 0
-    # planes of 64 nodes rounded up to 64, plus 64 elements of slack
-    assert nbytes.value == 19 * 128 * 4
+    # one 256-node tile of 19 planes, plus a slack tile on either side
+    assert nbytes.value == 19 * 3 * 256 * 4
 
 
 @pytest.mark.parametrize("dims,storage", [((4, 4, 4), 0), ((7, 9, 11), 1), ((64, 64, 64), 0), ((256, 256, 256), 1),
                                           ((255, 255, 255), 0)])
 def test_state_plane_has_slack_and_is_staggered(dims, storage):
-    """Every state plane holds more than the node count (the z-face corner
-    loads overreach by one element); large planes are not a power of two."""
+    """A state buffer holds whole tiles of the node count plus two slack
+    tiles (the z-face corner loads overreach by one node): more than n
+    elements per direction, 4096-node tiles on large grids."""
     lat = nat.Lattice()
     lat.q = 27
     lat.nx, lat.ny, lat.nz = dims
@@ -83,5 +84,5 @@ def test_state_plane_has_slack_and_is_staggered(dims, storage):
     n = dims[0] * dims[1] * dims[2]
     plane = nat.state_plane(lat)
     assert plane > n
-    if n >= 1 << 16:
-        assert plane % 256 == 0 and plane % 65536 != 0
+    tile = 4096 if n >= 1 << 20 else 256
+    assert plane == (-(-n // tile) + 2) * tile
