@@ -76,23 +76,23 @@ inline int64_t state_elems(int q, int64_t n, int ts) {
   return ((n + T - 1) / T + 2) * T * q;
 }
 
-// The pull-source table of KParams (soff, srk) for lattice L.
+// The pull-source table of KParams (soff, mixmask) for lattice L; false if
+// an offset does not fit 32 bits.
 template <class L>
-inline void fill_pull_table(KParams* p, int esz) {
+inline bool fill_pull_table(KParams* p, int esz) {
   const int64_t T = (int64_t)1 << p->ts;
-  for (int k = 0; k < 27; ++k) {
-    p->soff[k] = 0;
-    p->srk[k] = 0;
-  }
+  for (int k = 0; k < 27; ++k) p->soff[k] = 0;
   p->mixmask = 0;
   for (int k = 0; k < L::Q; ++k) {
     const int64_t sk = (int64_t)L::E[k][0] * p->sxy + (int64_t)L::E[k][1] * p->nz + L::E[k][2];
     const int64_t qk = sk >= 0 ? sk / T : -((-sk + T - 1) / T);  // floor
     const int64_t rk = sk - qk * T;
-    p->soff[k] = ((int64_t)L::Q - qk * L::Q + k) * T * esz - rk * esz;
-    p->srk[k] = (uint32_t)rk;
+    const int64_t so = ((int64_t)L::Q - qk * L::Q + k) * T * esz - rk * esz;
+    if (so > 0x7fffffffLL || so < -0x7fffffffLL) return false;
+    p->soff[k] = (int32_t)so;
     if (rk % 64 != 0) p->mixmask |= (uint64_t)1 << (rk / 64);  // waves with l0 < rk < l0 + 64
   }
+  return true;
 }
 
 // Validates a lattice descriptor and fills the kernel parameters.
@@ -116,16 +116,6 @@ inline int make_params(const funlbm_lattice* lat, KParams* p) {
   if (p->n > 0x7fffffffLL)
     return set_error(FUNLBM_ETOOBIG, "%lld nodes exceed the 2^31 - 1 node limit", (long long)p->n);
   p->sxy = lat->ny * lat->nz;
-  p->ts = state_tile_shift(p->n);
-  {
-    const int esz = storage_elem_bytes(lat->storage);
-    switch (lat->q) {
-      case 13: fill_pull_table<D3Q13>(p, esz); break;
-      case 15: fill_pull_table<D3Q15>(p, esz); break;
-      case 19: fill_pull_table<D3Q19>(p, esz); break;
-      default: fill_pull_table<D3Q27>(p, esz); break;
-    }
-  }
   p->has_moving = 0;
   for (int f = 0; f < 6; ++f) {
     const int bc = lat->bc[f];
@@ -146,6 +136,9 @@ inline int make_params(const funlbm_lattice* lat, KParams* p) {
   p->omega_d = 1.0 - 1.0 / (2.0 * p->tau_d);
   if (!magic_div(p->nz, &p->mz, &p->sz) || !magic_div(p->ny, &p->my, &p->sy))
     return set_error(FUNLBM_EINVAL, "grid extent without a 32-bit division multiplier");
+  if (dev && lat->rho0 != 1.0f)
+    return set_error(FUNLBM_EINVAL, "rho0 must be 1 (the deviation base of the 16-bit states is w_k), got %g",
+                     (double)lat->rho0);
   p->rho0 = lat->rho0;
   p->scale = dev ? lat->dev_scale : 1.0f;
   p->inv_scale = dev ? 1.0f / lat->dev_scale : 1.0f;
@@ -153,6 +146,20 @@ inline int make_params(const funlbm_lattice* lat, KParams* p) {
   for (int d = 0; d < 3; ++d) {
     p->g[d] = lat->body_force[d];
     p->has_body |= (lat->body_force[d] != 0.0f);
+  }
+  p->ts = state_tile_shift(p->n);
+  {
+    const int esz = storage_elem_bytes(lat->storage);
+    bool ok;
+    switch (lat->q) {
+      case 13: ok = fill_pull_table<D3Q13>(p, esz); break;
+      case 15: ok = fill_pull_table<D3Q15>(p, esz); break;
+      case 19: ok = fill_pull_table<D3Q19>(p, esz); break;
+      default: ok = fill_pull_table<D3Q27>(p, esz); break;
+    }
+    if (!ok)
+      return set_error(FUNLBM_ETOOBIG, "grid cross-section %d x %d too large for 32-bit pull offsets",
+                       lat->ny, lat->nz);
   }
   return 0;
 }

@@ -45,7 +45,7 @@ __global__ __launch_bounds__(kBlock) void k_stream(KParams p, const typename S::
     const Src s = source_of<L, K>(p, c);
     typename S::elem v = src[(int64_t)s.k * p.n + lin(p, s.x, s.y, s.z)];
     if (s.moving_face >= 0) {
-      const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
+      const float wr = wt<float>(L::W[K]);
       T val = S::load(v, wr, p.inv_scale, p.scale);
       val = val - moving_wall_term<L, K, T>(p, s.moving_face, (T)rho[node]);
       v = S::store(val, wr, p.inv_scale, p.scale);
@@ -105,7 +105,7 @@ __global__ __launch_bounds__(kBlock) void k_pack(KParams p, const float* __restr
   if (node >= p.n) return;
   static_for<0, L::Q>([&](auto kc) {
     constexpr int K = decltype(kc)::value;
-    const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
+    const float wr = wt<float>(L::W[K]);
     state[state_index<L::Q>(p, K, node)] =
         S::store((typename S::T)f[(int64_t)K * p.n + node], wr, p.inv_scale, p.scale);
   });
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_equilibrium(KParams p, const float* 
   collide<L, T>(f, uu, r, Relax<T>{(T)1, (T)1, (T)0}, e);
   static_for<0, L::Q>([&](auto kc) {
     constexpr int K = decltype(kc)::value;
-    const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
+    const float wr = wt<float>(L::W[K]);
     state[(int64_t)K * p.n + node] = S::store(e[K], wr, p.inv_scale, p.scale);
   });
 }
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_init_state(KParams p, typename S::el
   collide<L, T>(f, u, rho, relax_of<T>(p), (T*)nullptr);
   static_for<0, L::Q>([&](auto kc) {
     constexpr int K = decltype(kc)::value;
-    const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
+    const float wr = wt<float>(L::W[K]);
     state[state_index<L::Q>(p, K, node)] = S::store(f[K], wr, p.inv_scale, p.scale);
   });
 }

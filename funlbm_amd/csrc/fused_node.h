@@ -29,15 +29,13 @@ struct FusedIO {
   float* __restrict__ out_p;
 };
 
-// The post-step f, u, rho of a node (the reference's tensors after the step),
-// without the collision.  Shared by fused_node and the observers below.
+// Forcing (where FOL_new != 0, with u' from FOL_old) and the moments of a
+// node whose pulled populations are in f.
 template <class L, class S, bool FORCE>
-__device__ __forceinline__ void step_node(const KParams& p, int64_t node, const Node& c,
-                                          const FusedIO<S>& io, bool new_here, bool old_here,
-                                          typename S::T* f, typename S::T& rho,
-                                          typename S::T* u) {
+__device__ __forceinline__ void after_pull(const KParams& p, int64_t node, const Node& c,
+                                           const FusedIO<S>& io, bool new_here, bool old_here,
+                                           typename S::T* f, typename S::T& rho, typename S::T* u) {
   using T = typename S::T;
-  pull_node<L, S>(p, io.src, c, node, io.rho_prev, f);
   T Fn[3] = {(T)0, (T)0, (T)0};
   if (FORCE) {
 #pragma unroll
@@ -50,11 +48,56 @@ __device__ __forceinline__ void step_node(const KParams& p, int64_t node, const 
       for (int d = 0; d < 3; ++d)
         Fo[d] = (T)(old_here ? io.fol_old[(int64_t)d * p.n + node] + p.g[d] : p.g[d]);
       T r1, u1[3];
-      moments<L, T>(p, c, f, Fo, r1, u1);
-      add_forcing<L, T>(f, u1, Fn, relax_of<T>(p));
+      moments<L, T, S::fast>(p, c, f, Fo, r1, u1);
+      add_forcing<L, T, S::fast>(f, u1, Fn, relax_of<T>(p));
     }
   }
-  moments<L, T>(p, c, f, Fn, rho, u);
+  moments<L, T, S::fast>(p, c, f, Fn, rho, u);
+}
+
+// The post-step f, u, rho of a node (the reference's tensors after the step),
+// without the collision.  Shared by fused_node and the observers below.
+template <class L, class S, bool FORCE>
+__device__ __forceinline__ void step_node(const KParams& p, int64_t node, const Node& c,
+                                          const FusedIO<S>& io, bool new_here, bool old_here,
+                                          typename S::T* f, typename S::T& rho,
+                                          typename S::T* u) {
+  pull_node<L, S>(p, io.src, c, node, io.rho_prev, f);
+  after_pull<L, S, FORCE>(p, node, c, io, new_here, old_here, f, rho, u);
+}
+
+// Collision of a stepped node and the store of its post-collision
+// populations into the destination state (and, next to moving walls, its
+// density for the next step's wall term).
+template <class L, class S>
+__device__ __forceinline__ void collide_store(const KParams& p, int64_t node, const Node& c,
+                                              const FusedIO<S>& io, typename S::T* f, typename S::T rho,
+                                              const typename S::T* u) {
+  using T = typename S::T;
+  collide<L, T, S::fast>(f, u, rho, relax_of<T>(p), (T*)nullptr);
+  if (p.has_moving && io.rho_next) {
+    bool mf = false;
+#pragma unroll
+    for (int face = 0; face < 6; ++face)
+      mf |= (p.bc[face] == FUNLBM_BC_WALL_WITH_SPEED) && on_face(p, c, face);
+    if (mf) io.rho_next[node] = (float)rho;
+  }
+  // The state is written once and read back only by the next step, after
+  // every other plane has streamed through the caches: non-temporal stores.
+  // The wave's 64 nodes lie in one state tile (every caller runs waves of 64
+  // consecutive nodes from a multiple of 64): uniform tile base + in-tile
+  // byte offset, as in pull_node.
+  using R = typename S::raw;
+  constexpr int Q = L::Q;
+  const int ts = p.ts;
+  const uint32_t tmask = (1u << ts) - 1u;
+  const int tn = (__builtin_amdgcn_readfirstlane((int)node) & ~63) >> ts;
+  const uint32_t lb = ((uint32_t)node & tmask) * (uint32_t)sizeof(R);
+  static_for<0, Q>([&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    char* base = (char*)io.dst + ((int64_t)Q + (int64_t)tn * Q + K) * ((int64_t)1 << ts) * (int64_t)sizeof(R);
+    __builtin_nontemporal_store(S::store_raw(f[K], wt<float>(L::W[K]), p.inv_scale, p.scale), (R*)(base + lb));
+  });
 }
 
 template <class L, class S, bool FORCE, bool MAT>
@@ -78,31 +121,7 @@ __device__ __forceinline__ void fused_node(const KParams& p, int64_t node, const
     if (io.out_rho) io.out_rho[node] = (float)rho;
     if (io.out_p) io.out_p[node] = (float)(rho / (T)3);
   } else {
-    collide<L, T>(f, u, rho, relax_of<T>(p), (T*)nullptr);
-    if (p.has_moving && io.rho_next) {
-      bool mf = false;
-#pragma unroll
-      for (int face = 0; face < 6; ++face)
-        mf |= (p.bc[face] == FUNLBM_BC_WALL_WITH_SPEED) && on_face(p, c, face);
-      if (mf) io.rho_next[node] = (float)rho;
-    }
-    // The state is written once and read back only by the next step, after
-    // every other plane has streamed through the caches: non-temporal stores.
-    // The wave's 64 nodes lie in one state tile (every caller runs waves of 64
-    // consecutive nodes from a multiple of 64): uniform tile base + in-tile
-    // byte offset, as in pull_node.
-    using R = typename S::raw;
-    constexpr int Q = L::Q;
-    const int ts = p.ts;
-    const uint32_t tmask = (1u << ts) - 1u;
-    const int tn = (__builtin_amdgcn_readfirstlane((int)node) & ~63) >> ts;
-    const uint32_t lb = ((uint32_t)node & tmask) * (uint32_t)sizeof(R);
-    static_for<0, Q>([&](auto kc) {
-      constexpr int K = decltype(kc)::value;
-      const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
-      char* base = (char*)io.dst + ((int64_t)Q + (int64_t)tn * Q + K) * ((int64_t)1 << ts) * (int64_t)sizeof(R);
-      __builtin_nontemporal_store(S::store_raw(f[K], wr, p.inv_scale, p.scale), (R*)(base + lb));
-    });
+    collide_store<L, S>(p, node, c, io, f, rho, u);
   }
 }
 

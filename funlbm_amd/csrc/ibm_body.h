@@ -126,7 +126,7 @@ __device__ inline void interp_marker(const KParams& p, int m, int lane,
       for (int d = 0; d < 3; ++d)
         F[d] = (T)(fol_old ? fol_old[(int64_t)d * p.n + node] + p.g[d] : p.g[d]);
       T r, u[3];
-      moments<L, T>(p, c, f, F, r, u);
+      moments<L, T, S::fast>(p, c, f, F, r, u);
       rr = (float)r;
 #pragma unroll
       for (int d = 0; d < 3; ++d) uu[d] = (float)u[d];

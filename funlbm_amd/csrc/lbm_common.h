@@ -98,6 +98,7 @@ struct StoreF32 {
   using T = float;
   using elem = float;
   static constexpr int code = FUNLBM_STORE_F32;
+  static constexpr bool fast = false;  // the reference's float32 operations, bit for bit
   using raw = float;
   __device__ static inline float load(elem v, float, float, float) { return v; }
   __device__ static inline elem store(float f, float, float, float) { return f; }
@@ -109,6 +110,7 @@ struct StoreF64 {
   using T = double;
   using elem = double;
   static constexpr int code = FUNLBM_STORE_F64;
+  static constexpr bool fast = false;
   using raw = double;
   __device__ static inline double load(elem v, double, double, double) { return v; }
   __device__ static inline elem store(double f, double, double, double) { return f; }
@@ -120,8 +122,11 @@ struct StoreF16 {
   using T = float;
   using elem = __half;
   static constexpr int code = FUNLBM_STORE_F16;
+  // 11-bit storage: its rounding (2^-11 of the deviation) dominates the
+  // error, so the arithmetic uses reciprocals and FMAs (FastMath below)
+  static constexpr bool fast = true;
   __device__ static inline float load(elem v, float wrho0, float inv_scale, float) {
-    return __half2float(v) * inv_scale + wrho0;
+    return __builtin_fmaf(__half2float(v), inv_scale, wrho0);
   }
   __device__ static inline elem store(float f, float wrho0, float, float scale) {
     return __float2half_rn((f - wrho0) * scale);
@@ -140,8 +145,9 @@ struct StoreBF16 {
   using T = float;
   using elem = __hip_bfloat16;
   static constexpr int code = FUNLBM_STORE_BF16;
+  static constexpr bool fast = true;
   __device__ static inline float load(elem v, float wrho0, float inv_scale, float) {
-    return __bfloat162float(v) * inv_scale + wrho0;
+    return __builtin_fmaf(__bfloat162float(v), inv_scale, wrho0);
   }
   __device__ static inline elem store(float f, float wrho0, float, float scale) {
     return __float2bfloat16(((f - wrho0) * scale));
@@ -166,9 +172,9 @@ struct KParams {
   // Per direction k, the pull source node - s_k (s_k = e_k . (sxy, nz, 1))
   // split as s_k = q_k T + r_k, 0 <= r_k < T: soff[k] = byte offset of element
   // (k, tile -q_k, in-tile -r_k) from the state's tile 0 minus the slack, i.e.
-  // ((Q - q_k Q + k) T - r_k) * element size; srk[k] = r_k.
-  int64_t soff[27];
-  uint32_t srk[27];
+  // ((Q - q_k Q + k) T - r_k) * element size (|soff| < 2^31, checked on the
+  // host).  r_k * element size = (-soff[k]) mod (T * element size).
+  int32_t soff[27];
   // bit w: the sources of some direction straddle a tile boundary for the
   // wave at in-tile start 64 w (some r_k in (64 w, 64 w + 64))
   uint64_t mixmask;
@@ -380,7 +386,7 @@ __device__ inline void zface_pull(const KParams& p, const typename S::elem* __re
   static_for<0, L::Q>([&](auto kc) {
     constexpr int K = decltype(kc)::value;
     if constexpr (L::E[K][2] == DZ) {
-      const float wr = (float)(wt<float>(L::W[K]) * p.rho0);
+      const float wr = wt<float>(L::W[K]);
       if (bc == FUNLBM_BC_PERIODICAL) {
         const int64_t idx = state_index<L::Q>(p, K, node + (DZ > 0 ? (p.nz - 1) : -(p.nz - 1)));
         f[K] = S::load(src[idx], wr, p.inv_scale, p.scale);
@@ -425,24 +431,29 @@ __device__ inline void pull_node(const KParams& p, const typename S::elem* __res
       // plane K -> tile t - 1, plane K)
       const char* tb = (const char*)src + (int64_t)(nw >> ts) * Q * ((int64_t)sizeof(E) << ts);
       const uint32_t back = (uint32_t)((Q - 1) * ((int64_t)sizeof(E) << ts));
+      const uint32_t tbmask = ((uint32_t)sizeof(E) << ts) - 1u;  // bytes of one plane of a tile, less one
+      const uint32_t l0b = l0 * (uint32_t)sizeof(E);
       if (!((p.mixmask >> (l0 >> 6)) & 1)) {
         // every direction's sources lie in one tile: scalar base + the lane's
-        // in-tile offset, tile tn - q_K (rk <= l0) or tn - q_K - 1
+        // in-tile offset, tile tn - q_K (r_K <= l0) or tn - q_K - 1
         static_for<0, Q>([&](auto kc) {
           constexpr int K = decltype(kc)::value;
-          const char* ua = tb + p.soff[K];  // element (K, tile tn - q_K, in-tile -r_K)
-          const char* sb = p.srk[K] <= l0 ? ua : ua - back;
-          f[K] = S::load(*(const E*)(sb + lb), (float)(wt<float>(L::W[K]) * p.rho0), p.inv_scale, p.scale);
+          const int32_t so = p.soff[K];
+          const uint32_t rkb = (uint32_t)(-so) & tbmask;  // r_K in bytes
+          const char* ua = tb + so;  // element (K, tile tn - q_K, in-tile -r_K)
+          const char* sb = rkb <= l0b ? ua : ua - back;
+          f[K] = S::load(*(const E*)(sb + lb), wt<float>(L::W[K]), p.inv_scale, p.scale);
         });
       } else {
         // a wave at a tile edge: per lane, the source tile of a straddling direction
         static_for<0, Q>([&](auto kc) {
           constexpr int K = decltype(kc)::value;
-          const uint32_t rk = p.srk[K];
-          const char* ua = tb + p.soff[K];
-          const char* sb = rk <= l0 ? ua : ua - back;
-          const uint32_t vo = (rk > l0) & (rk < l0 + 64) ? lb + (local >= rk ? back : 0u) : lb;
-          f[K] = S::load(*(const E*)(sb + vo), (float)(wt<float>(L::W[K]) * p.rho0), p.inv_scale, p.scale);
+          const int32_t so = p.soff[K];
+          const uint32_t rkb = (uint32_t)(-so) & tbmask;
+          const char* ua = tb + so;
+          const char* sb = rkb <= l0b ? ua : ua - back;
+          const uint32_t vo = (rkb > l0b) & (rkb < l0b + 64u * (uint32_t)sizeof(E)) ? lb + (lb >= rkb ? back : 0u) : lb;
+          f[K] = S::load(*(const E*)(sb + vo), wt<float>(L::W[K]), p.inv_scale, p.scale);
         });
       }
     } else {
@@ -450,7 +461,7 @@ __device__ inline void pull_node(const KParams& p, const typename S::elem* __res
         constexpr int K = decltype(kc)::value;
         constexpr int ex = L::E[K][0], ey = L::E[K][1], ez = L::E[K][2];
         const int64_t sn = node - (ex * p.sxy + ey * p.nz + ez);
-        f[K] = S::load(src[state_index<Q>(p, K, sn)], (float)(wt<float>(L::W[K]) * p.rho0), p.inv_scale,
+        f[K] = S::load(src[state_index<Q>(p, K, sn)], wt<float>(L::W[K]), p.inv_scale,
                        p.scale);
       });
     }
@@ -464,7 +475,7 @@ __device__ inline void pull_node(const KParams& p, const typename S::elem* __res
     const int64_t idx = state_index<Q>(p, s.k, lin(p, s.x, s.y, s.z));
     // The element was stored with the weight of s.k; rev(k) and k share a
     // weight on every lattice.
-    T v = S::load(src[idx], (float)(wt<float>(L::W[K]) * p.rho0), p.inv_scale, p.scale);
+    T v = S::load(src[idx], wt<float>(L::W[K]), p.inv_scale, p.scale);
     if (s.moving_face >= 0) v = v - moving_wall_term<L, K, T>(p, s.moving_face, (T)rho_prev[node]);
     f[K] = v;
   });
@@ -474,7 +485,7 @@ __device__ inline void pull_node(const KParams& p, const typename S::elem* __res
 // Moments (src/funlbm/flow/d3/base.py:116-133): rho = sum_k f_k in direction
 // order; j = f.e in direction order; u = (j + F/2)/rho; u := 0 on masked faces.
 // ---------------------------------------------------------------------------
-template <class L, class T>
+template <class L, class T, bool FAST = false>
 __device__ inline void moments(const KParams& p, const Node& c, const T* f, const T F[3], T& rho,
                                T u[3]) {
   T r = (T)0, jx = (T)0, jy = (T)0, jz = (T)0;
@@ -489,10 +500,18 @@ __device__ inline void moments(const KParams& p, const Node& c, const T* f, cons
     if (L::E[k][2] == -1) jz = jz - f[k];
   }
   rho = r;
-  const T rr = (T)1 / r;
-  u[0] = divc(jx + F[0] / (T)2, r, rr);
-  u[1] = divc(jy + F[1] / (T)2, r, rr);
-  u[2] = divc(jz + F[2] / (T)2, r, rr);
+  if constexpr (FAST) {
+    // the hardware reciprocal (1 ulp) and products
+    const T rr = __builtin_amdgcn_rcpf(r);
+    u[0] = __builtin_fmaf(F[0], (T)0.5, jx) * rr;
+    u[1] = __builtin_fmaf(F[1], (T)0.5, jy) * rr;
+    u[2] = __builtin_fmaf(F[2], (T)0.5, jz) * rr;
+  } else {
+    const T rr = (T)1 / r;
+    u[0] = divc(jx + F[0] / (T)2, r, rr);
+    u[1] = divc(jy + F[1] / (T)2, r, rr);
+    u[2] = divc(jz + F[2] / (T)2, r, rr);
+  }
   if (uzero_at(p, c)) {
     u[0] = (T)0;
     u[1] = (T)0;
@@ -535,8 +554,31 @@ __device__ inline double sqrt_t<double>(double v) {
 // f_k <- f_k - (f_k - feq_k)/tau.  If feq != nullptr it receives feq_k.
 // e_rev(k) . u is exactly -(e_k . u), so tmp of a reverse direction is the
 // negated tmp of its partner.
-template <class L, class T>
+template <class L, class T, bool FAST = false>
 __device__ inline void collide(T* f, const T u[3], T rho, const Relax<T>& rx, T* feq) {
+  if constexpr (FAST) {
+    // the same equilibrium with 1/cs^2 = 3 and 1/cs^4 = 9 as products,
+    // |u|^2 without the square root, and the relaxation as one FMA:
+    // feq_k = (1 + t (1 + t/2) - 3/2 |u|^2) rho w_k, f_k += (feq_k - f_k)/tau
+    const T uu = __builtin_fmaf(u[2], u[2], __builtin_fmaf(u[1], u[1], u[0] * u[0]));
+    const T u2 = uu * (T)1.5;
+    T tmp[L::Q];
+    static_for<0, L::Q>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      if constexpr (L::REV[K] < K)
+        tmp[K] = -tmp[L::REV[K]];
+      else
+        tmp[K] = edotu<L, K, T>(u) * (T)3;
+    });
+    static_for<0, L::Q>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      const T t = tmp[K];
+      const T e = (__builtin_fmaf(t, __builtin_fmaf(t, (T)0.5, (T)1), (T)1) - u2) * (rho * wt<T>(L::W[K]));
+      if (feq) feq[K] = e;
+      f[K] = __builtin_fmaf(e - f[K], rx.rtau, f[K]);
+    });
+    return;
+  }
   const T nrm = sqrt_t<T>(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
   const T u2 = divc(nrm * nrm, Cs<T>::cs2(), Cs<T>::rcs2()) / (T)2;
   T tmp[L::Q];
@@ -559,9 +601,31 @@ __device__ inline void collide(T* f, const T u[3], T rho, const Relax<T>& rx, T*
 // Forcing term of FlowD3.cul_equ2 (src/funlbm/flow/d3/base.py:103-114):
 // t1 = (e_a - u)/cs^2, t2 = (u e_a) e_a / cs^4 (elementwise),
 // t3 = sum_d (t1 + t2)_d F_d, f_a += ((1 - 1/(2 tau)) w_a) t3.
-template <class L, class T>
+template <class L, class T, bool FAST = false>
 __device__ inline void add_forcing(T* f, const T u[3], const T F[3], const Relax<T>& rx) {
   const T omega = rx.omega;
+  if constexpr (FAST) {
+    // (t1 + t2)_d = 3 (e_d - u_d) + 9 u_d e_d^2: 3 e_d + 6 u_d where e_d != 0,
+    // -3 u_d where e_d = 0; so t3 = sum_d of A_d e_d + B_d or C_d
+    T A[3], B[3], C[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      A[d] = (T)3 * F[d];
+      B[d] = ((T)6 * u[d]) * F[d];
+      C[d] = ((T)-3 * u[d]) * F[d];
+    }
+    static_for<0, L::Q>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      T t3 = (T)0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int e = L::E[K][d];
+        t3 = t3 + (e == 0 ? C[d] : (e > 0 ? A[d] + B[d] : B[d] - A[d]));
+      }
+      f[K] = __builtin_fmaf(omega * wt<T>(L::W[K]), t3, f[K]);
+    });
+    return;
+  }
   static_for<0, L::Q>([&](auto kc) {
     constexpr int K = decltype(kc)::value;
     T t3 = (T)0;

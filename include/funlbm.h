@@ -59,7 +59,8 @@ typedef struct funlbm_lattice {
   int32_t bc[6];      /* FUNLBM_BC_* per face */
   int32_t uzero_mask; /* bit i: u := 0 on face i after every moment update (0x3f = reference) */
   float tau;          /* BGK relaxation time, 3*mu + 0.5 */
-  float rho0;         /* reference density of deviation storage (1.0) */
+  float rho0;         /* reference density of deviation storage: must be 1.0
+                         for F16/BF16 (the deviation base is w_k); ignored otherwise */
   float dev_scale;    /* scale of deviation storage (ignored for F32/F64) */
   float body_force[3];/* uniform force added to FOL everywhere (extension; 0 = none) */
   float uw[6][3];     /* wall velocity of WALL_WITH_SPEED faces */

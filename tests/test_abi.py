@@ -86,3 +86,23 @@ def test_state_plane_has_slack_and_is_staggered(dims, storage):
     assert plane > n
     tile = 4096 if n >= 1 << 20 else 256
     assert plane == (-(-n // tile) + 2) * tile
+
+
+def test_deviation_storage_needs_unit_rho0():
+    """The 16-bit states keep f - w_k (rho0 = 1); another rho0 is refused,
+    while float32/float64 ignore the field."""
+    lat = nat.Lattice()
+    lat.q = 19
+    lat.nx = lat.ny = lat.nz = 8
+    lat.tau = 0.8
+    lat.dev_scale = 1024.0
+    for i in range(6):
+        lat.bc[i] = nat.BC_PERIODICAL
+    nbytes = ctypes.c_int64()
+    lat.storage, lat.rho0 = nat.STORE_F16, 1.5
+    assert nat.lib().funlbm_state_bytes(ctypes.byref(lat), ctypes.byref(nbytes)) == nat.EINVAL
+    assert "rho0" in nat.last_error()
+    lat.rho0 = 1.0
+    assert nat.lib().funlbm_state_bytes(ctypes.byref(lat), ctypes.byref(nbytes)) == 0
+    lat.storage, lat.rho0 = nat.STORE_F32, 0.0
+    assert nat.lib().funlbm_state_bytes(ctypes.byref(lat), ctypes.byref(nbytes)) == 0

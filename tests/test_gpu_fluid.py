@@ -227,10 +227,14 @@ def run_oracle_rounded(q, size, mu, bcs, steps, f0, storage, body_force=None, sc
 @pytest.mark.parametrize("size", [(9, 10, 15), (11, 9, 13)])
 @pytest.mark.parametrize("storage", ["f16", "bf16"])
 @pytest.mark.parametrize("q", [19, 27])
-def test_deviation_storage_bitwise_equal_to_rounded_oracle(q, storage, size):
-    """f16/bf16 state: the fused path equals the oracle with the storage's
-    round trip applied after every collision, bit for bit (16 nodes along z
-    exercise the vector path, 14 the per-node path)."""
+def test_deviation_storage_matches_rounded_oracle(q, storage, size):
+    """f16/bf16 state: the fused path against the oracle with the storage's
+    round trip applied after every collision.  The 16-bit paths compute with
+    reciprocals and FMAs (float32 rounding, far below the storage's 2^-11 or
+    2^-8), so a rounding of the state can land on the other side now and then;
+    after 30 steps f stays within 2e-6 of max|f| for f16 (a flipped rounding
+    of a 1e-3 deviation moves f by ~5e-7 of max|f|) and 2e-5 for bf16 (8x
+    coarser)."""
     bcs = BC_SETS["channel"]
     nodes = tuple(s + 1 for s in size)
     f0 = random_state(q, nodes, seed=6)
@@ -238,4 +242,6 @@ def test_deviation_storage_bitwise_equal_to_rounded_oracle(q, storage, size):
     lbm = run_solver(q, size, 0.07, bcs, 30, f0=f0, storage=storage, body_force=g)
     st = run_oracle_rounded(q, size, 0.07, bcs, 30, f0, storage, body_force=g)
     got = lbm.flow.f.cpu().numpy()
-    assert np.array_equal(got.view(np.uint32), st.f.view(np.uint32)), rel_linf(got, st.f)
+    err = rel_linf(got, st.f)
+    print(storage, q, size, "rel L-inf", err)
+    assert err < {"f16": 2e-6, "bf16": 2e-5}[storage], err
