@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_fused(
     FusedIO<S> io{src, dst, fo, fn, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     with_lattice(cd.q, [&](auto l) {
       using L = decltype(l);
-      fused_node<L, S, true, false>(cd.p, node, io, new_here, old_here);
+      fused_step_node<L, S, true>(cd.p, node, io, new_here, old_here);
     });
   }
 }

@@ -185,7 +185,10 @@ __global__ __launch_bounds__(kBlock) void k_fused(
     old_here = old_here && (entry == stamp || entry == stamp - 1u);
   }
   FusedIO<S> io{src, dst, fol_old, fol_new, rho_prev, rho_next, out_f, out_u, out_rho, out_p};
-  fused_node<L, S, FORCE, MAT>(p, node, io, new_here, old_here);
+  if constexpr (MAT)
+    fused_node<L, S, FORCE, true>(p, node, io, new_here, old_here);
+  else
+    fused_step_node<L, S, FORCE>(p, node, io, new_here, old_here);
 }
 
 // Plain copy, one 16-byte element per thread (the fastest plain form measured

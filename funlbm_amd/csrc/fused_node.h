@@ -125,4 +125,61 @@ __device__ __forceinline__ void fused_node(const KParams& p, int64_t node, const
   }
 }
 
+// The fused step of a 16-bit state in deviation arithmetic (moments_dev,
+// add_forcing_dev, collide_dev): the populations stay scaled deviations from
+// the load to the store.
+template <class L, class S, bool FORCE>
+__device__ __forceinline__ void fused_node_dev(const KParams& p, int64_t node, const FusedIO<S>& io,
+                                               bool new_here, bool old_here) {
+  using D = DevOf<S>;
+  const Node c = node_coords(p, node);
+  float g[L::Q];
+  pull_node<L, D>(p, io.src, c, node, io.rho_prev, g);
+  float Fn[3] = {0.0f, 0.0f, 0.0f};
+  if (FORCE) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) Fn[d] = new_here ? io.fol_new[(int64_t)d * p.n + node] + p.g[d] : p.g[d];
+    if (Fn[0] != 0.0f || Fn[1] != 0.0f || Fn[2] != 0.0f) {
+      float Fo[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) Fo[d] = old_here ? io.fol_old[(int64_t)d * p.n + node] + p.g[d] : p.g[d];
+      float r1, u1[3];
+      moments_dev<L>(p, c, g, Fo, r1, u1);
+      add_forcing_dev<L>(p, g, u1, Fn);
+    }
+  }
+  float rho, u[3];
+  moments_dev<L>(p, c, g, Fn, rho, u);
+  collide_dev<L>(p, g, u, rho);
+  if (p.has_moving && io.rho_next) {
+    bool mf = false;
+#pragma unroll
+    for (int face = 0; face < 6; ++face)
+      mf |= (p.bc[face] == FUNLBM_BC_WALL_WITH_SPEED) && on_face(p, c, face);
+    if (mf) io.rho_next[node] = rho;
+  }
+  using R = typename S::raw;
+  constexpr int Q = L::Q;
+  const int ts = p.ts;
+  const uint32_t tmask = (1u << ts) - 1u;
+  const int tn = (__builtin_amdgcn_readfirstlane((int)node) & ~63) >> ts;
+  const uint32_t lb = ((uint32_t)node & tmask) * (uint32_t)sizeof(R);
+  static_for<0, Q>([&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    char* base = (char*)io.dst + ((int64_t)Q + (int64_t)tn * Q + K) * ((int64_t)1 << ts) * (int64_t)sizeof(R);
+    __builtin_nontemporal_store(S::from_float(g[K]), (R*)(base + lb));
+  });
+}
+
+// The fused step of one node: deviation arithmetic for the 16-bit states,
+// the reference's operations otherwise.
+template <class L, class S, bool FORCE>
+__device__ __forceinline__ void fused_step_node(const KParams& p, int64_t node, const FusedIO<S>& io,
+                                                bool new_here, bool old_here) {
+  if constexpr (S::fast)
+    fused_node_dev<L, S, FORCE>(p, node, io, new_here, old_here);
+  else
+    fused_node<L, S, FORCE, false>(p, node, io, new_here, old_here);
+}
+
 }  // namespace funlbm
