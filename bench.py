@@ -122,6 +122,65 @@ def profiled_traffic(storage, size, q):
     return None
 
 
+def storage_leg(size, q, storage, steps, warmup, ref_mlups=None):
+    """Step and kernel timing of one lattice/storage at size^3 (periodic, no particles)."""
+    import torch
+
+    lbm = build_solver(size, q, storage)
+    ms, _ = time_steps(lbm, steps, max(warmup, 50))
+    kms = time_fused_kernel(lbm)
+    n = lbm.flow.n
+    del lbm
+    torch.cuda.empty_cache()
+    elem = {"f32": 4, "f16": 2, "bf16": 2, "f64": 8}[storage]
+    mlups = n / (ms * 1e-3) / 1e6
+    ach = n * 2 * q * elem / (kms * 1e-3) / 1e9
+    out = {"workload": f"D3Q{q} {size}^3 periodic, tau 0.6, {storage} state", "mlups": round(mlups, 1),
+           "ms_per_step": round(ms, 4), "kernel_ms": round(kms, 4), "bytes_per_lu": 2 * q * elem,
+           "achieved_gbs": round(ach, 1), "frac_of_8tbs": round(ach / HBM_PEAK_GBS, 4)}
+    if ref_mlups:
+        out["speedup_vs_f32"] = round(mlups / ref_mlups, 3)
+    return out
+
+
+def bench_suspension(size=256, n_particles=64, steps=30, warmup=5, storage="f32"):
+    """The D3Q27 suspension: 64 prolate ellipsoids (4, 4, 8) at seeded
+    non-overlapping positions and random orientations in a periodic 256^3
+    box, whole LBMD3 steps (IBM included)."""
+    import torch
+
+    from funlbm_amd import precision
+    from funlbm_amd.lbm import LBMD3, Config
+
+    cfg = precision.suspension(n=size, n_particles=n_particles, mu=0.05 / 3)
+    cfg["flow"]["storage"] = storage
+    t0 = time.perf_counter()
+    lbm = LBMD3(config=Config().from_json(cfg))
+    lbm.init()
+    init_s = time.perf_counter() - t0
+    for k in range(1, warmup + 1):
+        lbm.step = k
+        lbm.run_step()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for k in range(warmup + 1, warmup + steps + 1):
+        lbm.step = k
+        lbm.run_step()
+    b.record()
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    n = lbm.flow.n
+    m = lbm.particle_swarm.n_markers
+    p = len(lbm.particle_swarm.particles)
+    del lbm
+    torch.cuda.empty_cache()
+    return {"workload": f"D3Q27 {size}^3 periodic, {p} ellipsoids (4,4,8), rho_p 1.2, body force 1e-6 x, "
+                        f"{storage} state, whole steps with IBM", "particles": p, "markers": m,
+            "ms_per_step": round(ms, 4), "mlups": round(n / (ms * 1e-3) / 1e6, 1), "init_s": round(init_s, 2)}
+
+
 def copy_bandwidth(nbytes=2 << 30, reps=10):
     """Measured copy peak: read + write bytes of a plain copy kernel / time."""
     import torch
@@ -276,7 +335,7 @@ def bench_serve(rate=20.0, duration=10.0, seed=0, steps=(1000, 20000), budget=20
     # error of the served storage against float64 on a sample request solved alone
     spec = reqs[0][1]
     fields = {}
-    for st in ("f64", storage):
+    for st in ("f64", "f32", storage):
         e = BatchEngine(storage=st, max_nodes=spec.n + 256, max_markers=sum(len(markers_for(p)[0]) for p in spec.particles) + 8,
                         max_particles=len(spec.particles) + 1, max_cases=1, max_q=27, traj_cap=8)
         cid = e.add(spec)
@@ -287,11 +346,17 @@ def bench_serve(rate=20.0, duration=10.0, seed=0, steps=(1000, 20000), budget=20
         torch.cuda.empty_cache()
     u64, c64 = fields["f64"]
     us, cs = fields[storage]
+    u32, c32 = fields["f32"]
     scale = float(np.abs(u64).max())
     stats["error_check"] = {
         "sample": f"request 0 alone: {spec.nodes[0]}^3 D3Q{spec.q}, {len(spec.particles)} particles, {spec.steps} steps",
+        "max_abs_u_f64": scale,
         "u_rel_linf_vs_f64": float(np.abs(us - u64).max() / scale) if scale > 0 else None,
         "centroid_abs_vs_f64": float(np.abs(cs - c64).max()) if len(c64) else None,
+        # float32 storage and arithmetic on the same request: at settling
+        # velocities (~1e-4) float32 rounding of the populations sets the error
+        "f32_u_rel_linf_vs_f64": float(np.abs(u32 - u64).max() / scale) if scale > 0 else None,
+        "f32_centroid_abs_vs_f64": float(np.abs(c32 - c64).max()) if len(c64) else None,
     }
     stats.update({
         "workload": f"Poisson {rate} req/s for {duration} s, sides 32-128, D3Q19/27, 1-4 particles, "
@@ -322,6 +387,10 @@ def main():
     ap.add_argument("--serve-steps", default="100,2000", help="min,max steps of a request")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-f16", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the bf16, D3Q27 and suspension legs")
+    ap.add_argument("--serve-rates", default="",
+                    help="comma-separated extra arrival rates (req/s) for a serving sweep")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -395,6 +464,16 @@ def main():
             "kernel_ms": round(k16, 4),
             "achieved_gbs": round(n * 2 * args.q * 2 / (k16 * 1e-3) / 1e9, 1),
         }
+    if not args.no_extra and args.storage == "f32":
+        try:
+            out["bf16_deviation"] = storage_leg(args.size, args.q, "bf16", args.steps, args.warmup, mlups)
+            q27 = storage_leg(args.size, 27, "f32", args.steps, args.warmup)
+            out["d3q27"] = {"f32": q27,
+                            "f16": storage_leg(args.size, 27, "f16", args.steps, args.warmup, q27["mlups"])}
+            out["suspension"] = bench_suspension(size=args.size)
+        except Exception as e:  # reported, never silent
+            out["extra_error"] = repr(e)
+        torch.cuda.empty_cache()
     if not args.no_batch:
         try:
             out["batched"] = bench_batch(n_cases=args.batch_cases, steps=args.batch_steps)
@@ -408,6 +487,15 @@ def main():
         except Exception as e:
             out["serving"] = {"error": repr(e)}
         torch.cuda.empty_cache()
+    if args.serve_rates:
+        lo, hi = (int(v) for v in args.serve_steps.split(","))
+        sweep = []
+        for r in (float(v) for v in args.serve_rates.split(",")):
+            s = bench_serve(rate=r, duration=args.serve_seconds, steps=(lo, hi))
+            sweep.append({k: s[k] for k in ("rate", "completed", "solves_per_s", "p50_latency_s", "p99_latency_s",
+                                            "busy_s", "wall_s")})
+            torch.cuda.empty_cache()
+        out["serving_sweep"] = sweep
     if not args.no_copy:
         cb = copy_bandwidth()
         out["roofline"]["copy_peak_gbs"] = round(cb, 1)

@@ -168,13 +168,20 @@ __global__ __launch_bounds__(kBlock) void k_init_state(KParams p, typename S::el
 // ---------------------------------------------------------------------------
 // Fused pull step, one node per thread (body in fused_node.h).
 // ---------------------------------------------------------------------------
+//
+// The parameter block is read through the kernel-argument segment pointer (it
+// is the first argument, at offset 0): a runtime index into it (the moving
+// wall's face) is then a load from that segment, where with the by-value
+// argument the compiler may copy the whole block to private memory.
 template <class L, class S, bool FORCE, bool MAT>
 __global__ __launch_bounds__(kBlock) void k_fused(
-    KParams p, const typename S::elem* __restrict__ src, typename S::elem* __restrict__ dst,
+    KParams p_arg, const typename S::elem* __restrict__ src, typename S::elem* __restrict__ dst,
     const float* __restrict__ fol_old, const float* __restrict__ fol_new,
     const uint32_t* __restrict__ fol_tiles, const int* __restrict__ ctl,
     const float* __restrict__ rho_prev, float* __restrict__ rho_next, float* __restrict__ out_f,
     float* __restrict__ out_u, float* __restrict__ out_rho, float* __restrict__ out_p) {
+  (void)p_arg;
+  const KParams& p = *(const KParams*)__builtin_amdgcn_kernarg_segment_ptr();
   const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
   const int64_t node = (int64_t)b * kBlock + threadIdx.x;
   if (node >= p.n) return;
