@@ -60,7 +60,9 @@ __device__ inline void with_lattice(int q, Fn&& fn) {
 // ---------------------------------------------------------------------------
 // Fused step over the active tiles (256 nodes of one case each).
 // ---------------------------------------------------------------------------
-template <class S>
+// One kernel per lattice (L), each taking the tiles of its own cases: a
+// single kernel over all four lattices carries the registers of the largest.
+template <class S, class L>
 __global__ __launch_bounds__(kBlock) void k_batch_fused(
     const CaseDesc* __restrict__ cases, const int* __restrict__ tile_case,
     const int* __restrict__ tile_local, const int* __restrict__ hdr, BatchDyn dyn,
@@ -73,6 +75,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_fused(
     const int c = tile_case[t];
     if (c < 0 || dyn.left[c] <= 0) continue;
     const CaseDesc& cd = cases[c];
+    if (cd.q != L::Q) continue;
     const int lt = tile_local[t];
     const int64_t node = (int64_t)lt * kBlock + threadIdx.x;
     if (node >= cd.p.n) continue;
@@ -89,10 +92,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_fused(
     const bool new_here = entry == stamp;
     const bool old_here = entry == stamp || entry == stamp - 1u;
     FusedIO<S> io{src, dst, fo, fn, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    with_lattice(cd.q, [&](auto l) {
-      using L = decltype(l);
-      fused_step_node<L, S, true>(cd.p, node, io, new_here, old_here);
-    });
+    fused_step_node<L, S, true>(cd.p, node, io, new_here, old_here);
   }
 }
 
@@ -322,10 +322,15 @@ int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, in
                          origin_ok, fol_a, fol_b, (const unsigned long long*)owner, fol_tiles);
       if ((rc = check_launch("batch_spread_write"))) return rc;
     }
-    hipLaunchKernelGGL((k_batch_fused<S>), dim3(grid_tiles), dim3(kBlock), 0, s, cd, tile_case,
-                       tile_local, hdr, dyn, (E*)pool_a, (E*)pool_b, (const float*)fol_a,
-                       (const float*)fol_b, (const uint32_t*)fol_tiles);
-    if ((rc = check_launch("batch_fused"))) return rc;
+    auto fused = [&](auto l) {
+      using L = decltype(l);
+      hipLaunchKernelGGL((k_batch_fused<S, L>), dim3(grid_tiles), dim3(kBlock), 0, s, cd, tile_case,
+                         tile_local, hdr, dyn, (E*)pool_a, (E*)pool_b, (const float*)fol_a,
+                         (const float*)fol_b, (const uint32_t*)fol_tiles);
+      return check_launch("batch_fused");
+    };
+    if ((rc = fused(D3Q13{})) || (rc = fused(D3Q15{})) || (rc = fused(D3Q19{})) || (rc = fused(D3Q27{})))
+      return rc;
     if (with_ibm) {
       hipLaunchKernelGGL(k_batch_pose, dim3(grid_particles), dim3(kIbmBlock), 0, s, hdr, dyn,
                          particle_case, particle_markers, lag, lx, pstate, traj, traj_cap);
