@@ -20,6 +20,7 @@ c_int64 = ctypes.c_int64
 BC_PERIODICAL = 11000
 BC_WALL = 1200
 BC_WALL_WITH_SPEED = 1201
+BC_NON_EQUILIBRIUM = 1400
 
 STORE_F32 = 0
 STORE_F16 = 1
@@ -74,8 +75,9 @@ SIGNATURES = {
     "funlbm_last_error": (ctypes.c_char_p, []),
     "funlbm_state_bytes": (c_int32, [LATP, ctypes.POINTER(c_int64)]),
     "funlbm_collide": (c_int32, [LATP, P, P, P, P, P]),
-    "funlbm_stream": (c_int32, [LATP, P, P, P, P]),
+    "funlbm_stream": (c_int32, [LATP, P, P, P, P, P]),
     "funlbm_moments": (c_int32, [LATP, P, P, P, P, P, P]),
+    "funlbm_neq_moments": (c_int32, [LATP, P, P, P, ctypes.c_float, P, P]),
     "funlbm_forcing": (c_int32, [LATP, P, P, P, P]),
     "funlbm_pack": (c_int32, [LATP, P, P, P]),
     "funlbm_equilibrium_state": (c_int32, [LATP, P, P, P, P]),

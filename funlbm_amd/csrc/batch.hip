@@ -261,6 +261,7 @@ int funlbm_case_pack(const funlbm_lattice* lat, int64_t f_offset, int64_t node_o
   memset(&d, 0, sizeof(d));
   int rc = make_params(lat, &d.p);
   if (rc) return rc;
+  if ((rc = reject_neq(d.p, "case_pack"))) return rc;
   if (d.p.has_moving)
     return set_error(FUNLBM_EUNSUPPORTED_BC, "batched cases take PERIODICAL and WALL faces only");
   if (f_offset < 0 || node_offset < 0 || tile_begin < 0)

@@ -117,15 +117,20 @@ inline int make_params(const funlbm_lattice* lat, KParams* p) {
     return set_error(FUNLBM_ETOOBIG, "%lld nodes exceed the 2^31 - 1 node limit", (long long)p->n);
   p->sxy = lat->ny * lat->nz;
   p->has_moving = 0;
+  p->has_neq = 0;
   for (int f = 0; f < 6; ++f) {
     const int bc = lat->bc[f];
-    if (bc != FUNLBM_BC_PERIODICAL && bc != FUNLBM_BC_WALL && bc != FUNLBM_BC_WALL_WITH_SPEED)
+    const bool neq_ok = bc == FUNLBM_BC_NON_EQUILIBRIUM && f < 2;  // the reference's x faces only
+    if (bc != FUNLBM_BC_PERIODICAL && bc != FUNLBM_BC_WALL && bc != FUNLBM_BC_WALL_WITH_SPEED && !neq_ok)
       return set_error(FUNLBM_EUNSUPPORTED_BC, "boundary code %d on face %d is not implemented",
                        bc, f);
     p->bc[f] = bc;
     p->has_moving |= (bc == FUNLBM_BC_WALL_WITH_SPEED);
+    p->has_neq |= neq_ok;
     for (int d = 0; d < 3; ++d) p->uw[f][d] = lat->uw[f][d];
   }
+  if (p->has_neq && lat->nx < 3)
+    return set_error(FUNLBM_EINVAL, "NON_EQUILIBRIUM faces need nx >= 3, got %d", lat->nx);
   p->uzero = lat->uzero_mask & 0x3f;
   p->tau = lat->tau;
   // the reference's float32 arithmetic on the host: correctly rounded
@@ -161,6 +166,14 @@ inline int make_params(const funlbm_lattice* lat, KParams* p) {
       return set_error(FUNLBM_ETOOBIG, "grid cross-section %d x %d too large for 32-bit pull offsets",
                        lat->ny, lat->nz);
   }
+  return 0;
+}
+
+// The fused state paths (fused step, materialise, statistics, batched cases,
+// interpolation from the state) do not implement NON_EQUILIBRIUM faces.
+inline int reject_neq(const KParams& p, const char* what) {
+  if (p.has_neq)
+    return set_error(FUNLBM_EUNSUPPORTED_BC, "%s: NON_EQUILIBRIUM faces run on the explicit path", what);
   return 0;
 }
 

@@ -35,7 +35,8 @@ __global__ __launch_bounds__(kBlock) void k_collide(KParams p, float* __restrict
 template <class L, class S>
 __global__ __launch_bounds__(kBlock) void k_stream(KParams p, const typename S::elem* __restrict__ src,
                                                    typename S::elem* __restrict__ dst,
-                                                   const float* __restrict__ rho) {
+                                                   const float* __restrict__ rho,
+                                                   const float* __restrict__ feq) {
   using T = typename S::T;
   const int64_t node = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (node >= p.n) return;
@@ -43,6 +44,20 @@ __global__ __launch_bounds__(kBlock) void k_stream(KParams p, const typename S::
   static_for<0, L::Q>([&](auto kc) {
     constexpr int K = decltype(kc)::value;
     const Src s = source_of<L, K>(p, c);
+    if (s.neq != 0) {
+      // f = feq + (f_streamed - feq) of the next plane in (float32 state):
+      // the streamed value there is the plain roll of x_in - e_k (periodic)
+      constexpr int ex = L::E[K][0], ey = L::E[K][1], ez = L::E[K][2];
+      const int xin = s.neq > 0 ? 1 : p.nx - 2;
+      const int sx = xin - ex;  // in [0, nx) for nx >= 3
+      const int sy = (c.y - ey + p.ny) % p.ny;
+      const int sz = (c.z - ez + p.nz) % p.nz;
+      const int64_t kn = (int64_t)K * p.n;
+      const float fin = (float)src[kn + lin(p, sx, sy, sz)];
+      const float v = feq[kn + node] + (fin - feq[kn + lin(p, xin, c.y, c.z)]);
+      dst[kn + node] = (typename S::elem)v;
+      return;
+    }
     typename S::elem v = src[(int64_t)s.k * p.n + lin(p, s.x, s.y, s.z)];
     if (s.moving_face >= 0) {
       const float wr = wt<float>(L::W[K]);
@@ -52,6 +67,44 @@ __global__ __launch_bounds__(kBlock) void k_stream(KParams p, const typename S::
     }
     dst[(int64_t)K * p.n + node] = v;
   });
+}
+
+// NON_EQUILIBRIUM moments: the max of rho (positive floats order as their
+// bit patterns), then the face overrides of update_u_rou_boundary.
+__global__ __launch_bounds__(kBlock) void k_fill_zero(float* __restrict__ x) { x[0] = 0.0f; }
+
+__global__ __launch_bounds__(kBlock) void k_rho_max(int64_t n, const float* __restrict__ rho,
+                                                    unsigned int* __restrict__ out) {
+  float m = 0.0f;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    m = fmaxf(m, rho[i]);
+  __shared__ float red[kBlock];
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(out, __float_as_uint(red[0]));
+}
+
+__global__ __launch_bounds__(kBlock) void k_neq_apply(KParams p, float* __restrict__ u, float* __restrict__ rho,
+                                                      const double* __restrict__ profile, float remu,
+                                                      const float* __restrict__ rho_max) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;  // (y, z) of a face plane
+  if (i >= (int64_t)p.sxy) return;
+  const int64_t in0 = i, in1 = (int64_t)p.sxy + i;                                  // x = 0, 1
+  const int64_t out0 = (int64_t)(p.nx - 1) * p.sxy + i, out1 = (int64_t)(p.nx - 2) * p.sxy + i;
+  if (p.bc[0] == FUNLBM_BC_NON_EQUILIBRIUM) {
+    const float uw = (remu / rho_max[0]) / (float)min(p.ny, p.nz);
+    u[in0] = (float)((double)uw * profile[i]);
+    rho[in0] = rho[in1];
+  }
+  if (p.bc[1] == FUNLBM_BC_NON_EQUILIBRIUM) {
+    rho[out0] = rho[out1];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[(int64_t)d * p.n + out0] = u[(int64_t)d * p.n + out1];
+  }
 }
 
 template <class L>
@@ -245,13 +298,15 @@ int funlbm_collide(const funlbm_lattice* lat, float* f, const float* u, const fl
 }
 
 int funlbm_stream(const funlbm_lattice* lat, const void* f_src, void* f_dst, const float* rho,
-                  void* stream) {
+                  const float* feq, void* stream) {
   KParams p;
   int rc = make_params(lat, &p);
   if (rc) return rc;
   if (!f_src || !f_dst) return set_error(FUNLBM_EINVAL, "stream: NULL array");
   if (f_src == f_dst) return set_error(FUNLBM_EINVAL, "stream: source and destination alias");
   if (p.has_moving && !rho) return set_error(FUNLBM_EINVAL, "stream: moving wall needs rho");
+  if (p.has_neq && (!feq || lat->storage != FUNLBM_STORE_F32))
+    return set_error(FUNLBM_EINVAL, "stream: NON_EQUILIBRIUM faces need feq and float32 storage");
   hipStream_t s = (hipStream_t)stream;
   return dispatch_lattice(lat->q, [&](auto l) {
     using L = decltype(l);
@@ -259,7 +314,7 @@ int funlbm_stream(const funlbm_lattice* lat, const void* f_src, void* f_dst, con
       using S = decltype(st);
       using E = typename S::elem;
       hipLaunchKernelGGL((k_stream<L, S>), dim3(grid_for(p.n)), dim3(kBlock), 0, s, p,
-                         (const E*)f_src, (E*)f_dst, rho);
+                         (const E*)f_src, (E*)f_dst, rho, feq);
       return check_launch("stream");
     });
   });
@@ -277,6 +332,28 @@ int funlbm_moments(const funlbm_lattice* lat, const float* f, const float* fol, 
     hipLaunchKernelGGL(k_moments<L>, dim3(grid_for(p.n)), dim3(kBlock), 0, s, p, f, fol, u, rho, pr);
     return check_launch("moments");
   });
+}
+
+int funlbm_neq_moments(const funlbm_lattice* lat, float* u, float* rho, const double* profile, float remu,
+                       float* work, void* stream) {
+  KParams p;
+  int rc = make_params(lat, &p);
+  if (rc) return rc;
+  if (!p.has_neq) return 0;
+  if (!u || !rho || !work) return set_error(FUNLBM_EINVAL, "neq_moments: NULL array");
+  const bool inlet = p.bc[0] == FUNLBM_BC_NON_EQUILIBRIUM;
+  if (inlet && !profile) return set_error(FUNLBM_EINVAL, "neq_moments: an input NEQ face needs the profile");
+  hipStream_t s = (hipStream_t)stream;
+  if (inlet) {
+    hipLaunchKernelGGL(k_fill_zero, dim3(1), dim3(1), 0, s, work);
+    if ((rc = check_launch("neq_moments"))) return rc;
+    const int blocks = (int)std::min<int64_t>((p.n + kBlock - 1) / kBlock, 2048);
+    hipLaunchKernelGGL(k_rho_max, dim3(blocks), dim3(kBlock), 0, s, p.n, (const float*)rho, (unsigned int*)work);
+    if ((rc = check_launch("neq_moments"))) return rc;
+  }
+  hipLaunchKernelGGL(k_neq_apply, dim3(grid_for(p.sxy)), dim3(kBlock), 0, s, p, u, rho, profile, remu,
+                     (const float*)work);
+  return check_launch("neq_moments");
 }
 
 int funlbm_forcing(const funlbm_lattice* lat, float* f, const float* u, const float* fol,
@@ -332,6 +409,7 @@ int funlbm_init_state(const funlbm_lattice* lat, void* state, void* stream) {
   KParams p;
   int rc = make_params(lat, &p);
   if (rc) return rc;
+  if ((rc = reject_neq(p, "init_state"))) return rc;
   if (!state) return set_error(FUNLBM_EINVAL, "init_state: NULL state");
   hipStream_t s = (hipStream_t)stream;
   return dispatch_lattice(lat->q, [&](auto l) {
@@ -352,6 +430,7 @@ static int fused_launch(const funlbm_lattice* lat, const void* src, void* dst, c
   KParams p;
   int rc = make_params(lat, &p);
   if (rc) return rc;
+  if ((rc = reject_neq(p, mat ? "materialize" : "fused_step"))) return rc;
   if (!src || (!mat && !dst)) return set_error(FUNLBM_EINVAL, "fused: NULL state");
   if (!mat && src == dst) return set_error(FUNLBM_EINVAL, "fused: source and destination alias");
   if (p.has_moving && !rho_prev)

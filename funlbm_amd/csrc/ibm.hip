@@ -94,6 +94,7 @@ int funlbm_ibm_interp(const funlbm_lattice* lat, const void* state, const float*
   if (n_markers < 0) return set_error(FUNLBM_EINVAL, "ibm_interp: negative marker count");
   if (n_markers == 0) return 0;
   const bool from_state = state != nullptr;
+  if (from_state && (rc = reject_neq(p, "ibm_interp from a state"))) return rc;
   if (!from_state && (!u || !rho))
     return set_error(FUNLBM_EINVAL, "ibm_interp: give either a state or u and rho");
   if (!lx || !marker_particle || !pstate || !lu || !lF || !ctl)

@@ -210,6 +210,7 @@ struct KParams {
   int bc[6];
   int uzero;
   int has_moving;  // any WALL_WITH_SPEED face
+  int has_neq;     // a NON_EQUILIBRIUM input/output face (explicit path only)
   int has_body;    // body_force != 0
   float tau;       // float32 tau as the reference's field
   float rtau;      // float32(1 / tau)
@@ -332,6 +333,7 @@ __device__ inline bool uzero_at(const KParams& p, const Node& c) {
 struct Src {
   int x, y, z, k;
   int moving_face;  // -1 or the face whose wall velocity corrects the value
+  int neq;          // 1 / -1: the input / output NON_EQUILIBRIUM rule gives the value
 };
 
 template <class L, int K>
@@ -340,6 +342,7 @@ __device__ inline Src source_of(const KParams& p, const Node& c) {
   Src s;
   s.k = K;
   s.moving_face = -1;
+  s.neq = 0;
   s.x = c.x - ex;
   s.y = c.y - ey;
   s.z = c.z - ez;
@@ -352,6 +355,8 @@ __device__ inline Src source_of(const KParams& p, const Node& c) {
 
   auto apply = [&](int face, int ox, int oy, int oz) {
     const int bc = p.bc[face];
+    if (bc == FUNLBM_BC_NON_EQUILIBRIUM) return;  // set for every direction below
+    s.neq = 0;
     if (bc == FUNLBM_BC_PERIODICAL) {
       s.x = ox; s.y = oy; s.z = oz; s.k = K; s.moving_face = -1;
     } else {  // WALL or WALL_WITH_SPEED (others are rejected on the host)
@@ -359,8 +364,12 @@ __device__ inline Src source_of(const KParams& p, const Node& c) {
       s.moving_face = (bc == FUNLBM_BC_WALL_WITH_SPEED) ? face : -1;
     }
   };
+  // NON_EQUILIBRIUM input/output: every direction of the face plane, in the
+  // reference's order (input, then output), before the y and z faces
+  if (c.x == 0 && p.bc[0] == FUNLBM_BC_NON_EQUILIBRIUM) s.neq = 1;
   if (ex == 1 && c.x == 0) apply(0, p.nx - 1, c.y, c.z);
   if (ex == -1 && c.x == p.nx - 1) apply(1, 0, c.y, c.z);
+  if (c.x == p.nx - 1 && p.bc[1] == FUNLBM_BC_NON_EQUILIBRIUM) s.neq = -1;
   if (ey == 1 && c.y == 0) apply(2, c.x, p.ny - 1, c.z);
   if (ey == -1 && c.y == p.ny - 1) apply(3, c.x, 0, c.z);
   if (ez == 1 && c.z == 0) apply(4, c.x, c.y, p.nz - 1);

@@ -107,6 +107,7 @@ int funlbm_flow_stats(const funlbm_lattice* lat, const void* state_src, const fl
   KParams p;
   int rc = make_params(lat, &p);
   if (rc) return rc;
+  if ((rc = reject_neq(p, "flow_stats"))) return rc;
   if (!state_src || !partials || !out) return set_error(FUNLBM_EINVAL, "flow_stats: NULL array");
   if (n_partials < 1) return set_error(FUNLBM_EINVAL, "flow_stats: need at least one partial");
   if (p.has_moving && !rho_prev)

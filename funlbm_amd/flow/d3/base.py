@@ -87,6 +87,11 @@ class FlowD3(Flow):
         return lat
 
     @property
+    def has_neq(self) -> bool:
+        """A NON_EQUILIBRIUM input/output face (explicit path only)."""
+        return any(b.is_condition(BoundaryCondition.NON_EQUILIBRIUM) for b in self.config.boundary.faces())
+
+    @property
     def has_moving_wall(self) -> bool:
         return any(b.is_condition(BoundaryCondition.WALL_WITH_SPEED) for b in self.config.boundary.faces())
 
@@ -121,6 +126,18 @@ class FlowD3(Flow):
         self._mat_valid = True
         self._pristine = True
         self._x = None
+        self._neq_profile = None
+        self._neq_work = None
+        if self.has_neq:
+            from .neq import duct_profile
+
+            if self.storage != "f32":
+                raise NotImplementedError("NON_EQUILIBRIUM faces run on the explicit float32 path")
+            self._neq_work = torch.zeros((1,), device=dev, dtype=_F32)
+            self._neq_remu = float(np.float32(self.config.Re * self.config.mu))
+            if self.config.boundary.input.is_condition(BoundaryCondition.NON_EQUILIBRIUM):
+                prof = duct_profile(self.nodes[1], self.nodes[2])
+                self._neq_profile = torch.from_numpy(prof.reshape(-1)).to(dev)
         # f = rho w with rho = 1 (exactly w_k in float32)
         nat.call("funlbm_equilibrium_state", self.lat32, None, None, nat.ptr(self._f), self._stream())
         self._initialised = True
@@ -149,10 +166,19 @@ class FlowD3(Flow):
             "funlbm_moments", self.lat32, nat.ptr(self._f), self._fol_ptr(self._fc),
             nat.ptr(self._u), nat.ptr(self._rho), nat.ptr(self._p), self._stream(),
         )
+        self.update_u_rou_boundary()
         self._pristine = False
 
     def update_u_rou_boundary(self, *args, **kwargs):
-        """Part of update_u_rou's kernel (the face mask); kept for the API."""
+        """The face zeroing is part of the moments kernel; the NON_EQUILIBRIUM
+        overrides (inlet duct profile scaled by Re mu / max(rho) / min(Y, Z),
+        rho copied from the next plane in; outlet rho and u copied) run here
+        (src/funlbm/flow/d3/base.py:135-148)."""
+        if self.has_neq:
+            nat.call(
+                "funlbm_neq_moments", self.lat32, nat.ptr(self._u), nat.ptr(self._rho),
+                nat.ptr(self._neq_profile), self._neq_remu, nat.ptr(self._neq_work), self._stream(),
+            )
 
     def cul_equ(self, step=0, *args, **kwargs):
         """BGK collision, keeps feq (src/funlbm/flow/d3/base.py:150-159)."""
@@ -169,9 +195,11 @@ class FlowD3(Flow):
         self._ensure_explicit()
         if self._ftmp is None:
             self._ftmp = torch.empty_like(self._f)
+        if self.has_neq and self._feq is None:
+            raise RuntimeError("NON_EQUILIBRIUM streaming needs the equilibrium of a previous cul_equ")
         nat.call(
             "funlbm_stream", self.lat32, nat.ptr(self._f), nat.ptr(self._ftmp), nat.ptr(self._rho),
-            self._stream(),
+            nat.ptr(self._feq) if self.has_neq else None, self._stream(),
         )
         self._f, self._ftmp = self._ftmp, self._f
 
@@ -203,6 +231,8 @@ class FlowD3(Flow):
         followed by the conversion of f* into the state storage."""
         if self._mode == "fused":
             return
+        if self.has_neq:
+            raise NotImplementedError("NON_EQUILIBRIUM faces run on the explicit path (config fused: false)")
         self._alloc_state()
         s = self._stream()
         if self._pristine and self.lat.storage != nat.STORE_F32:

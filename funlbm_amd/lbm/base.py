@@ -107,7 +107,8 @@ class LBMBase(Worker):
 
     def run_step(self, *args, **kwargs) -> None:
         """src/funlbm/lbm/base.py:146-157."""
-        if self.config.fused:
+        # NON_EQUILIBRIUM faces are implemented on the explicit path only
+        if self.config.fused and not getattr(self.flow, "has_neq", False):
             self._run_step_fused()
         else:
             self._compute_flow()

@@ -34,6 +34,7 @@ extern "C" {
 #define FUNLBM_BC_PERIODICAL 11000
 #define FUNLBM_BC_WALL 1200
 #define FUNLBM_BC_WALL_WITH_SPEED 1201
+#define FUNLBM_BC_NON_EQUILIBRIUM 1400 /* input/output faces, explicit entry points only */
 
 /* Face order of bc[], uw[] and the bits of uzero_mask: the reference's
  * input(x=0), output(x=X), back(y=0), forward(y=Y), bottom(z=0), top(z=Z)
@@ -90,9 +91,13 @@ int funlbm_collide(const funlbm_lattice* lat, float* f, const float* u, const fl
 /* Streaming with the six face overrides, replaces FlowD3.f_stream and
  * f_stream_bound* (src/funlbm/flow/d3/base.py:161-323).  Works on the storage
  * format of lat->storage, so it is a bit-exact permutation for PERIODICAL and
- * WALL faces.  rho (float, [x][y][z]) is read only on WALL_WITH_SPEED faces. */
+ * WALL faces.  rho (float, [x][y][z]) is read only on WALL_WITH_SPEED faces.
+ * feq ([q][x][y][z] float, the last collision's equilibrium) is read only on
+ * NON_EQUILIBRIUM faces, where every direction becomes feq + (f - feq) of the
+ * next plane in (src/funlbm/flow/d3/base.py:214-218,237-241); those faces
+ * need float32 storage and nx >= 3. */
 int funlbm_stream(const funlbm_lattice* lat, const void* f_src, void* f_dst, const float* rho,
-                  void* stream);
+                  const float* feq, void* stream);
 
 /* Moments, replaces FlowD3.update_u_rou + update_u_rou_boundary
  * (src/funlbm/flow/d3/base.py:116-133): rho = sum f, p = rho/3,
@@ -100,6 +105,17 @@ int funlbm_stream(const funlbm_lattice* lat, const void* f_src, void* f_dst, con
  * fol may be NULL (zero); body_force is added to fol.  p may be NULL. */
 int funlbm_moments(const funlbm_lattice* lat, const float* f, const float* fol, float* u,
                    float* rho, float* p, void* stream);
+
+/* The NON_EQUILIBRIUM overrides of update_u_rou_boundary
+ * (src/funlbm/flow/d3/base.py:135-148), run after funlbm_moments: on an
+ * input NEQ face u_x = uw * profile[y][z] (double, the duct profile of
+ * init_u for u_max = 1, [ny][nz]) with uw = float32(remu / max(rho)) /
+ * min(ny, nz) in float32 (remu = float32(Re * mu)), the other components
+ * stay zero, and rho = rho of the next plane in; on an output NEQ face
+ * rho and u are copied from the next plane in.  work: 1 float of device
+ * scratch (the max).  No-op without NEQ faces. */
+int funlbm_neq_moments(const funlbm_lattice* lat, float* u, float* rho, const double* profile, float remu,
+                       float* work, void* stream);
 
 /* Forcing term, replaces FlowD3.cul_equ2 (src/funlbm/flow/d3/base.py:103-114):
  * f_a += (1 - 1/(2 tau)) w_a sum_d [(e_ad - u_d)/cs^2 + u_d e_ad e_ad/cs^4] FOL_d. */

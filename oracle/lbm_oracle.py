@@ -99,8 +99,11 @@ class Lattice:
 class FlowState:
     """The reference Flow's tensors (src/funlbm/flow/base.py:141-157), AoS."""
 
-    def __init__(self, lat: Lattice, nodes, tau, bc, uw=None, uzero_mask=0x3F, body_force=None):
+    def __init__(self, lat: Lattice, nodes, tau, bc, uw=None, uzero_mask=0x3F, body_force=None, Re=10.0,
+                 mu=None):
         self.lat = lat
+        self.Re = float(Re)
+        self.mu = float(mu) if mu is not None else float((float(tau) - 0.5) / 3.0)
         self.dt = lat.dtype
         self.nodes = tuple(int(n) for n in nodes)
         self.tau = self.dt.type(tau)
@@ -123,14 +126,14 @@ class FlowState:
 
 
 def init_flow(q, nodes, mu=None, tau=None, bc=(WALL,) * 6, dtype=np.float32, uw=None,
-              uzero_mask=0x3F, body_force=None):
+              uzero_mask=0x3F, body_force=None, Re=10.0):
     """FlowD3.init (src/funlbm/flow/d3/base.py:72-101): u = 0, rho = 1,
     tau = 3 mu + 0.5, FOL = 0, f = rho w, then update_u_rou.  The extra
     cul_equ of LBMD3._init (src/funlbm/lbm/lbm3d.py:29) is init_solver's."""
     lat = Lattice(q, dtype)
     if tau is None:
         tau = np.float32(3 * mu + 0.5) if lat.dtype == np.float32 else 3 * mu + 0.5
-    st = FlowState(lat, nodes, tau, bc, uw=uw, uzero_mask=uzero_mask, body_force=body_force)
+    st = FlowState(lat, nodes, tau, bc, uw=uw, uzero_mask=uzero_mask, body_force=body_force, Re=Re, mu=mu)
     st.f = (st.rou * lat.w[None, None, None, :]).astype(st.dt)
     update_u_rou(st)
     return st
@@ -171,12 +174,58 @@ def moments(lat, f, F, uzero):
     return r[..., None], u
 
 
+def duct_series(y, z, ha, hb, n_terms=100):
+    """cul_u (src/funlbm/flow/d3/base.py:10-25) at one point, unrounded:
+    1 - (y/ha)^2 + sum_n 32 (-1)^(n+1) / (k^3 pi^3) cos(k pi y / (2 ha))
+    cosh(k pi z / (2 ha)) / cosh(k pi hb / (2 ha)), k = 2n + 1."""
+    n = np.arange(n_terms)
+    k = 2 * n + 1
+    base = 32 * (-1) ** (n + 1) / (k ** 3 * np.pi ** 3)
+    y_term = np.cos(k * np.pi * y / (2 * ha))
+    z_term = np.cosh(k * np.pi * z / (2 * ha)) / np.cosh(k * np.pi * hb / (2 * ha))
+    return 1 - (y / ha) ** 2 + np.sum(base * y_term * z_term)
+
+
+def duct_profile(a, b, n_terms=100):
+    """init_u's table for u_max = 1 (src/funlbm/flow/d3/base.py:28-37): the
+    series at y = linspace(-(a-1)/2, (a-1)/2, a), z likewise over b, half
+    widths a/2 and b/2, each value rounded to 6 decimals."""
+    y = np.linspace(-(a - 1) / 2, (a - 1) / 2, a)
+    z = np.linspace(-(b - 1) / 2, (b - 1) / 2, b)
+    out = np.empty((a, b))
+    for i, yy in enumerate(y):
+        for j, zz in enumerate(z):
+            out[i, j] = round(duct_series(yy, zz, a / 2.0, b / 2.0, n_terms), 6)
+    return out
+
+
+def neq_moments(st: FlowState):
+    """The NON_EQUILIBRIUM overrides of update_u_rou_boundary
+    (src/funlbm/flow/d3/base.py:135-148): inlet u_x = uw * profile with
+    uw = Re mu / max(rho) / min(Y, Z) (float32), rho[0] = rho[1]; outlet
+    rho[-1] = rho[-2], u[-1] = u[-2].  The reference passes uw to init_u as
+    a tensor and then calls torch.from_numpy on the tensor product, which
+    raises; the profile is applied here as that code intends."""
+    dt = st.dt
+    if st.bc[0] == NON_EQUILIBRIUM:
+        shape = st.u.shape
+        uw = dt.type(st.Re * st.mu) / st.rou.max()
+        uw = dt.type(uw / dt.type(min(shape[1], shape[2])))
+        prof = duct_profile(shape[1], shape[2])
+        st.u[0, :, :, 0] = (np.float64(uw) * prof).astype(dt)
+        st.rou[0] = st.rou[1]
+    if st.bc[1] == NON_EQUILIBRIUM:
+        st.rou[-1] = st.rou[-2]
+        st.u[-1] = st.u[-2]
+
+
 def update_u_rou(st: FlowState):
-    """src/funlbm/flow/d3/base.py:116-133 (PERIODICAL/WALL faces)."""
+    """src/funlbm/flow/d3/base.py:116-148."""
     rho, u = moments(st.lat, st.f, st.fol_eff(), face_mask(st.nodes, st.uzero_mask))
     st.rou = rho
     st.p = rho / st.dt.type(3.0)
     st.u = u
+    neq_moments(st)
 
 
 def _edotu(lat, k, u):
@@ -243,7 +292,13 @@ def f_stream(st: FlowState):
             key[axis] = sl
             return arr[tuple(key)]
 
-        if bc == PERIODICAL:
+        if bc == NON_EQUILIBRIUM and axis == 0:
+            # all directions: feq + the non-equilibrium part of the next
+            # plane in, whose values are the streamed (and, for the outlet,
+            # already inlet-overridden) ones of self.f
+            nxt = slice(1, 2) if sign == 1 else slice(-2, -1)
+            f[here] = st.feq[here] + (f[nxt] - st.feq[nxt])
+        elif bc == PERIODICAL:
             at(here, f)[..., index] = at(there, fcopy)[..., index]
         elif bc == WALL:
             at(here, f)[..., index] = at(here, fcopy)[..., rev]

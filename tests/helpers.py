@@ -2,7 +2,7 @@
 
 import numpy as np
 
-BC_NAMES = {11000: "PERIODICAL", 1200: "WALL", 1201: "WALL_WITH_SPEED"}
+BC_NAMES = {11000: "PERIODICAL", 1200: "WALL", 1201: "WALL_WITH_SPEED", 1400: "NON_EQUILIBRIUM"}
 FACES = ("input", "output", "back", "forward", "bottom", "top")
 
 

@@ -4,6 +4,8 @@ Streaming and bounce-back are permutations and must be bit-exact; the float
 arithmetic must agree with the oracle within 1e-5 relative L-inf after the
 step counts below (the reference's own criterion for 1000 steps)."""
 
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -248,3 +250,47 @@ def test_deviation_storage_matches_rounded_oracle(q, storage, size):
     tol = 2.0 * 2.0 ** -bits * dev / float(np.max(np.abs(st.f)))
     print(storage, q, size, "rel L-inf", err, "tolerance", tol)
     assert err < tol, (err, tol)
+
+
+@pytest.mark.parametrize("q", [19, 27])
+def test_neq_faces_match_oracle_bitwise(q):
+    """NON_EQUILIBRIUM inlet and outlet (explicit path, chosen by LBMD3 for
+    such faces): the duct-profile inlet, the copied outlet and the
+    feq + (f - feq) streaming rule, bit for bit with the oracle over 25 steps."""
+    N = orc.NON_EQUILIBRIUM
+    size = (8, 7, 9)
+    bcs = [N, N, W, W, P, P]
+    nodes = tuple(s + 1 for s in size)
+    f0 = random_state(q, nodes, seed=12)
+    from funlbm_amd.lbm import LBMD3, Config
+
+    flow = flow_json(q, size, 0.05, bcs)
+    flow["param"]["Re"] = 40.0
+    lbm = LBMD3(config=Config().from_json({"flow": flow, "log_every": 0}))
+    lbm.init()
+    lbm.flow.f = torch.from_numpy(f0)
+    lbm.flow.update_u_rou()
+    st = orc.init_flow(q, nodes, mu=0.05, bc=bcs, Re=40.0)
+    orc.init_solver(st)
+    st.f = f0.copy()
+    orc.update_u_rou(st)
+    for s in range(1, 26):
+        lbm.step = s
+        lbm.run_step()
+        orc.fluid_step(st)
+    got = lbm.flow.f.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), st.f.view(np.uint32)), rel_linf(got, st.f)
+    u = lbm.flow.u.cpu().numpy()
+    assert np.array_equal(u.view(np.uint32), st.u.view(np.uint32)), rel_linf(u, st.u)
+    assert float(np.abs(u[0, :, :, 0]).max()) > 0  # the inlet profile is there
+
+
+def test_neq_faces_refused_by_fused_entry_points():
+    from funlbm_amd import _native as nat
+
+    N = orc.NON_EQUILIBRIUM
+    fl = make_flow(19, (6, 5, 7), 0.05, [N, N, W, W, P, P])
+    with pytest.raises(NotImplementedError):
+        fl.begin_step()
+    rc = nat.lib().funlbm_fused_step(ctypes.byref(fl.lat), None, None, None, None, None, None, None, None, None)
+    assert rc == nat.EUNSUPPORTED_BC
