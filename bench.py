@@ -59,23 +59,29 @@ def build_solver(n, q, storage, tau=0.6, seed=0):
     return lbm
 
 
-def time_steps(lbm, steps, warmup):
+def time_steps(lbm, steps, warmup, repeats=5):
+    """Per-step time of exactly `steps` timed steps (after `warmup` untimed
+    ones), split into `repeats` consecutive chunks each bracketed by HIP events
+    on the launch stream; the median chunk's per-step time."""
     import torch
 
     for _ in range(warmup):
         lbm.run_step()
     torch.cuda.synchronize()
-    start = torch.cuda.Event(enable_timing=True)
-    stop = torch.cuda.Event(enable_timing=True)
-    start.record()
+    repeats = max(1, min(repeats, steps))
+    chunks = [steps // repeats] * repeats
+    chunks[-1] += steps - sum(chunks)
+    events = [torch.cuda.Event(enable_timing=True) for _ in range(repeats + 1)]
     t0 = time.perf_counter()
-    for _ in range(steps):
-        lbm.run_step()
-    stop.record()
+    events[0].record()
+    for i, c in enumerate(chunks):
+        for _ in range(c):
+            lbm.run_step()
+        events[i + 1].record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ms = start.elapsed_time(stop)
-    return ms / steps, wall
+    runs = [events[i].elapsed_time(events[i + 1]) / c for i, c in enumerate(chunks)]
+    return float(np.median(runs)), wall
 
 
 def time_fused_kernel(lbm, reps=50):

@@ -360,6 +360,12 @@ class FlowD3(Flow):
         self._pristine = False
         return self._view(self._fol[self._fc], 3)
 
+    def fol_array(self):
+        """A copy of the current FOL, [X, Y, Z, 3], without marking it written."""
+        if not self._fol_live[self._fc] or self._fol[self._fc] is None:
+            return torch.zeros((*self.nodes, 3), device=self.device, dtype=_F32)
+        return self._view(self._fol[self._fc], 3).clone()
+
     @FOL.setter
     def FOL(self, value):
         v = torch.as_tensor(value, device=self.device, dtype=_F32).reshape(*self.nodes, 3)

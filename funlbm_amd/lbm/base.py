@@ -18,6 +18,11 @@ class Config(BaseConfig):
                  the reference's method sequence, one kernel per method
       log_every  cadence of the per-step statistics of run() (default 1, the
                  reference's; 0 disables them)
+      checkpoint_every  write a checkpoint every this many steps into
+                 file.cache_dir/checkpoint (default 0: none; the reference
+                 writes every file.per_steps = 100 steps, 1.3 GB each for a
+                 256^3 D3Q19 lattice)
+    The reference's `checkpoint` key (a file to resume from) is honoured.
     """
 
     def __init__(self, *args, **kwargs):
@@ -32,6 +37,7 @@ class Config(BaseConfig):
         self.particles: List = []
         self.fused: bool = True
         self.log_every: int = 1
+        self.checkpoint_every: int = 0
 
     def _from_json(self, config_json: dict, *args, **kwargs) -> "Config":
         from ..particle import ParticleConfig
@@ -49,6 +55,7 @@ class Config(BaseConfig):
         self.fused = True if fused is None else bool(fused)
         le = deep_get(config_json, "log_every")
         self.log_every = 1 if le is None else int(le)
+        self.checkpoint_every = int(deep_get(config_json, "checkpoint_every") or 0)
         return self
 
 
@@ -140,7 +147,11 @@ class LBMBase(Worker):
         raise NotImplementedError()
 
     def init(self, *args, **kwargs) -> None:
+        """src/funlbm/lbm/base.py:181-184: set up, then resume from
+        config.checkpoint if given."""
         self._init()
+        if self.config.checkpoint is not None:
+            self.load_checkpoint(self.config.checkpoint)
 
     def _init(self, *args, **kwargs):
         raise NotImplementedError()
@@ -155,6 +166,28 @@ class LBMBase(Worker):
         raise NotImplementedError()
 
     def save(self, step=10, *args, **kwargs):
-        """Checkpoint output of the reference (src/funlbm/lbm/base.py:198-203)
-        is not written by this package."""
-        return None
+        """Checkpoint every config.checkpoint_every steps
+        (src/funlbm/lbm/base.py:198-203); the file is
+        checkpoint_{step:010d}.npz in file.cache_dir/checkpoint."""
+        every = self.config.checkpoint_every
+        if not every or step % every:
+            return None
+        path = f"{self.config.file_config.checkpoint_path}/checkpoint_{str(step).zfill(10)}.npz"
+        return self.dump_checkpoint(path)
+
+    def dump_checkpoint(self, path: str) -> str:
+        """src/funlbm/lbm/base.py:205-210."""
+        from .checkpoint import dump
+
+        return dump(self, path)
+
+    def load_checkpoint(self, checkpoint_path: str = None, *args, **kwargs) -> None:
+        """src/funlbm/lbm/base.py:212-227."""
+        import os
+
+        from .checkpoint import load
+
+        if checkpoint_path is None or not os.path.exists(checkpoint_path):
+            logger.error("load failed: checkpoint %r not found", checkpoint_path)
+            return
+        load(self, checkpoint_path)
