@@ -18,6 +18,8 @@ class Config(BaseConfig):
                  the reference's method sequence, one kernel per method
       log_every  cadence of the per-step statistics of run() (default 1, the
                  reference's; 0 disables them)
+      db_path    SQLite file for the per-step records of run() (the
+                 reference's flow / particle tables; default none)
       checkpoint_every  write a checkpoint every this many steps into
                  file.cache_dir/checkpoint (default 0: none; the reference
                  writes every file.per_steps = 100 steps, 1.3 GB each for a
@@ -38,6 +40,7 @@ class Config(BaseConfig):
         self.fused: bool = True
         self.log_every: int = 1
         self.checkpoint_every: int = 0
+        self.db_path = None
 
     def _from_json(self, config_json: dict, *args, **kwargs) -> "Config":
         from ..particle import ParticleConfig
@@ -56,6 +59,7 @@ class Config(BaseConfig):
         le = deep_get(config_json, "log_every")
         self.log_every = 1 if le is None else int(le)
         self.checkpoint_every = int(deep_get(config_json, "checkpoint_every") or 0)
+        self.db_path = deep_get(config_json, "db_path") or self.db_path
         return self
 
 
@@ -83,6 +87,11 @@ class LBMBase(Worker):
         )
         self.run_status = True
         self.history = []
+        self.store = None
+        if self.config.db_path:
+            from .store import StepStore
+
+            self.store = StepStore(self.config.db_path)
 
     def run(self, max_steps: int = 1000000, *args, **kwargs) -> None:
         """src/funlbm/lbm/base.py:86-103."""
@@ -107,7 +116,14 @@ class LBMBase(Worker):
         rec = {"step": self.step, "flow": self.flow.to_json()}
         rec["particles"] = [p.to_json() for p in self.particle_swarm.particles]
         self.history.append(rec)
-        logger.info("step=%6d %s", self.step, rec["flow"])
+        if self.store is not None:
+            self.store.put(self.step, rec["flow"], rec["particles"])
+        fl = rec["flow"]
+        info = [f"step={self.step:6d}"] + [
+            f"{k}=" + ",".join(f"{v:.6f}" for v in fl[k]) for k in ("f", "u", "rho")
+        ]
+        info += [p.to_str(self.step) for p in self.particle_swarm.particles]
+        logger.info("\t".join(info))
 
     def _check_run_status(self):
         pass

@@ -64,3 +64,19 @@ def test_periodic_checkpoints_are_written(tmp_path):
     advance(lbm, 1, 7)
     files = sorted(p.name for p in (tmp_path / "checkpoint").iterdir())
     assert files == ["checkpoint_0000000003.npz", "checkpoint_0000000006.npz"]
+
+
+def test_run_writes_step_records(tmp_path):
+    from funlbm_amd.lbm.store import StepStore
+
+    sphere = [{"type": "sphere", "r": 3, "rou": 1.5, "coord": {"center": [5, 5, 6]}}]
+    db = str(tmp_path / "run.db")
+    lbm = solver(sphere, db_path=db, log_every=2, max_step=6)
+    lbm.run()
+    lbm.store.close()
+    s = StepStore(db)
+    for step in (2, 4, 6):
+        rec = s.flow(step)
+        assert set(rec) == {"f", "u", "rho"} and len(rec["rho"]) == 3
+        assert "cx" in s.particle(step, 1)
+    assert s.flow(3) is None
