@@ -449,7 +449,7 @@ def main():
             "bytes_per_lu": bytes_per_lu,
             "bytes_per_launch": n * bytes_per_lu,
             "kernel_ms": round(kernel_ms, 4),
-            "kernel": "k_fused (funlbm_fused_step)",
+            "kernel": "k_fused_step (funlbm_fused_step)",
         },
     }
     del lbm

@@ -227,13 +227,12 @@ __global__ __launch_bounds__(kBlock) void k_init_state(KParams p, typename S::el
 // wall's face) is then a load from that segment, where with the by-value
 // argument the compiler may copy the whole block to private memory.
 template <class L, class S, bool FORCE, bool MAT>
-__global__ __launch_bounds__(kBlock) void k_fused(
-    KParams p_arg, const typename S::elem* __restrict__ src, typename S::elem* __restrict__ dst,
+__device__ __forceinline__ void fused_kernel_body(
+    const typename S::elem* __restrict__ src, typename S::elem* __restrict__ dst,
     const float* __restrict__ fol_old, const float* __restrict__ fol_new,
     const uint32_t* __restrict__ fol_tiles, const int* __restrict__ ctl,
     const float* __restrict__ rho_prev, float* __restrict__ rho_next, float* __restrict__ out_f,
     float* __restrict__ out_u, float* __restrict__ out_rho, float* __restrict__ out_p) {
-  (void)p_arg;
   const KParams& p = *(const KParams*)__builtin_amdgcn_kernarg_segment_ptr();
   const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
   const int64_t node = (int64_t)b * kBlock + threadIdx.x;
@@ -249,6 +248,31 @@ __global__ __launch_bounds__(kBlock) void k_fused(
     fused_node<L, S, FORCE, true>(p, node, io, new_here, old_here);
   else
     fused_step_node<L, S, FORCE>(p, node, io, new_here, old_here);
+}
+
+#define FUNLBM_FUSED_ARGS                                                                              \
+  KParams p_arg, const typename S::elem *__restrict__ src, typename S::elem *__restrict__ dst,        \
+      const float *__restrict__ fol_old, const float *__restrict__ fol_new,                            \
+      const uint32_t *__restrict__ fol_tiles, const int *__restrict__ ctl,                             \
+      const float *__restrict__ rho_prev, float *__restrict__ rho_next, float *__restrict__ out_f,     \
+      float *__restrict__ out_u, float *__restrict__ out_rho, float *__restrict__ out_p
+
+template <class L, class S, bool FORCE, bool MAT>
+__global__ __launch_bounds__(kBlock) void k_fused(FUNLBM_FUSED_ARGS) {
+  (void)p_arg;
+  fused_kernel_body<L, S, FORCE, MAT>(src, dst, fol_old, fol_new, fol_tiles, ctl, rho_prev, rho_next, out_f,
+                                      out_u, out_rho, out_p);
+}
+
+// The time step proper of the float32 and 16-bit states: at least seven
+// waves per SIMD (at most 72 VGPRs).  Left alone the compiler gives the
+// D3Q27 16-bit step 74 (six waves); with the bound it fits 63.
+template <class L, class S, bool FORCE>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_fused_step(
+    FUNLBM_FUSED_ARGS) {
+  (void)p_arg;
+  fused_kernel_body<L, S, FORCE, false>(src, dst, fol_old, fol_new, fol_tiles, ctl, rho_prev, rho_next, out_f,
+                                        out_u, out_rho, out_p);
 }
 
 // Plain copy, one 16-byte element per thread (the fastest plain form measured
@@ -450,7 +474,10 @@ static int fused_launch(const funlbm_lattice* lat, const void* src, void* dst, c
         return check_launch(mat ? "materialize" : "fused_step");
       };
       if (mat) return force ? go(k_fused<L, S, true, true>) : go(k_fused<L, S, false, true>);
-      return force ? go(k_fused<L, S, true, false>) : go(k_fused<L, S, false, false>);
+      if constexpr (S::code == FUNLBM_STORE_F64)
+        return force ? go(k_fused<L, S, true, false>) : go(k_fused<L, S, false, false>);
+      else
+        return force ? go(k_fused_step<L, S, true>) : go(k_fused_step<L, S, false>);
     });
   });
 }

@@ -724,21 +724,24 @@ __device__ inline void collide_dev(const KParams& p, float* g, const float u[3],
   const float uu = __builtin_fmaf(u[2], u[2], __builtin_fmaf(u[1], u[1], u[0] * u[0]));
   const float u2 = uu * 1.5f;
   const float sr = p.scale * p.rtau;  // s / tau
-  float tmp[L::Q];
-  static_for<0, L::Q>([&](auto kc) {
+  // a direction and its reverse together: t_rev = -t, so no table of t is kept
+  auto relax = [&](auto kc, float t) {
     constexpr int K = decltype(kc)::value;
-    if constexpr (L::REV[K] < K)
-      tmp[K] = -tmp[L::REV[K]];
-    else
-      tmp[K] = edotu<L, K, float>(u) * 3.0f;
-  });
-  static_for<0, L::Q>([&](auto kc) {
-    constexpr int K = decltype(kc)::value;
-    const float t = tmp[K];
     const float a = __builtin_fmaf(t, __builtin_fmaf(t, 0.5f, 1.0f), 1.0f) - u2;
     // g += (g_eq - g) / tau, g_eq = s w_k (rho a - 1)
     const float geq_tau = (sr * wt<float>(L::W[K])) * __builtin_fmaf(rho, a, -1.0f);
     g[K] = __builtin_fmaf(-g[K], p.rtau, g[K]) + geq_tau;
+  };
+  static_for<0, L::Q>([&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    constexpr int R = L::REV[K];
+    if constexpr (R == K) {
+      relax(kc, 0.0f);
+    } else if constexpr (R > K) {
+      const float t = edotu<L, K, float>(u) * 3.0f;
+      relax(kc, t);
+      relax(std::integral_constant<int, R>{}, -t);
+    }
   });
 }
 
