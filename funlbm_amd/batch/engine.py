@@ -291,16 +291,22 @@ class BatchEngine:
         """Rebuilds the active tile list and the header counts (host -> device)."""
         if not self._dirty:
             return
+        # tiles ordered by lattice (D3Q13, 15, 19, 27): one fused kernel per
+        # lattice takes its own range of the list
         tc, tl = [], []
-        for cid, s in sorted(self.slots.items()):
-            tc.append(np.full(s.n_tiles, cid, np.int32))
-            tl.append(np.arange(s.n_tiles, dtype=np.int32))
-        nt = int(sum(len(a) for a in tc))
+        ends = []
+        for q in (13, 15, 19, 27):
+            for cid, s in sorted(self.slots.items()):
+                if s.spec.q == q:
+                    tc.append(np.full(s.n_tiles, cid, np.int32))
+                    tl.append(np.arange(s.n_tiles, dtype=np.int32))
+            ends.append(int(sum(len(a) for a in tc)))
+        nt = ends[-1]
         if nt:
             self.tile_case[:nt] = torch.from_numpy(np.concatenate(tc)).to(self.device)
             self.tile_local[:nt] = torch.from_numpy(np.concatenate(tl)).to(self.device)
         stamp = int(self.hdr[H_STAMP].item())
-        hdr = [nt, self.markers.high_water, self.parts.high_water, self.max_cases, stamp, 0, 0, 0]
+        hdr = [nt, self.markers.high_water, self.parts.high_water, self.max_cases, stamp] + ends[:3]
         self.hdr.copy_(torch.tensor(hdr, dtype=torch.int32))
         self._with_ibm = 1 if self.markers.high_water > 0 else 0
         self._dirty = False

@@ -27,6 +27,8 @@ enum : int {
   H_PARTICLES = 2,  // particles in the particle arrays
   H_CASES = 3,      // cases in the case arrays
   H_STAMP = 4,      // stamp of the last completed spreading
+  H_QEND = 5,       // [5], [6], [7]: end of the D3Q13, D3Q15, D3Q19 tiles (the list is
+                    // ordered by lattice; D3Q27 tiles run to hdr[H_TILES])
   H_WORDS = 8
 };
 
@@ -69,9 +71,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_fused(
     typename S::elem* pool_a, typename S::elem* pool_b, const float* fol_a, const float* fol_b,
     const uint32_t* __restrict__ fol_tiles) {
   using E = typename S::elem;
-  const int n_tiles = hdr[H_TILES];
+  constexpr int li = L::Q == 13 ? 0 : L::Q == 15 ? 1 : L::Q == 19 ? 2 : 3;
+  const int t_begin = li == 0 ? 0 : hdr[H_QEND + li - 1];
+  const int t_end = li == 3 ? hdr[H_TILES] : hdr[H_QEND + li];
   const uint32_t stamp = (uint32_t)hdr[H_STAMP] + 1u;  // this step's spreading
-  for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+  for (int t = t_begin + (int)blockIdx.x; t < t_end; t += gridDim.x) {
     const int c = tile_case[t];
     if (c < 0 || dyn.left[c] <= 0) continue;
     const CaseDesc& cd = cases[c];

@@ -252,7 +252,10 @@ int funlbm_case_pack(const funlbm_lattice* lat, int64_t f_offset, int64_t node_o
  * counters advance (parity flips, left -1, done +1) and hdr[4] (the spreading
  * stamp) +1.  hdr is int32[8] device memory: [0] active tiles in
  * tile_case/tile_local, [1] marker slots, [2] particle slots, [3] case slots,
- * [4] stamp.  A slot whose case index is -1 is skipped.  particle_markers holds
+ * [4] stamp, [5], [6], [7] the ends of the D3Q13, D3Q15 and D3Q19 tiles: the
+ * tile list holds the tiles of D3Q13 cases first, then D3Q15, D3Q19 and D3Q27
+ * ones (one fused kernel per lattice takes its range).  A slot whose case
+ * index is -1 is skipped.  particle_markers holds
  * [begin, end) marker ranges (two int32 per particle).  traj (may be NULL)
  * receives float[4] = (cx, cy, cz, step) per particle and step in a ring of
  * traj_cap[0] entries.  The grid sizes fix the persistent launch shapes. */
