@@ -5,6 +5,7 @@ into funlbm_amd/_lib/libfunlbm_hip.so, which the ctypes loader opens.
 """
 
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -45,11 +46,48 @@ def sources():
     )
 
 
+# Kernels whose register allocation decides the step's speed: the build
+# fails if the compiler spills them to private memory (the allocation is
+# sensitive to unrelated code; a spill costs several times the step time).
+HOT_KERNELS = re.compile(r"k_fused_step|k_batch_fused")
+RESOURCE_REMARKS = ["-Rpass-analysis=kernel-resource-usage"]
+
+
+def resource_report(stderr: str):
+    """{kernel: {field: value}} from the kernel-resource-usage remarks."""
+    out, name = {}, None
+    for line in stderr.splitlines():
+        m = re.search(r"remark:\s+Function Name: (\S+)", line)
+        if m:
+            name = m.group(1)
+            out[name] = {}
+            continue
+        m = re.search(r"remark:\s+([^:]+): (\S+) \[-Rpass-analysis", line)
+        if m and name:
+            out[name][m.group(1).strip()] = m.group(2)
+    return out
+
+
+def check_hot_kernels(report, src):
+    bad = []
+    for name, r in report.items():
+        if not HOT_KERNELS.search(name):
+            continue
+        # (scalar spills go to vector-register lanes, not to memory)
+        scratch = int(r.get("ScratchSize [bytes/lane]", "0"))
+        spills = int(r.get("VGPRs Spill", "0"))
+        if scratch or spills:
+            bad.append(f"{name}: scratch {scratch} B/lane, VGPR spills {spills}")
+    if bad:
+        raise RuntimeError(f"hot kernels of {src} spill to private memory:\n" + "\n".join(bad))
+
+
 def _compile(src, obj):
-    cmd = [hipcc(), *HIPCC_FLAGS, "-c", src, "-o", obj]
+    cmd = [hipcc(), *HIPCC_FLAGS, *RESOURCE_REMARKS, "-c", src, "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src}:\n{res.stderr}")
+    check_hot_kernels(resource_report(res.stderr), src)
     return obj
 
 

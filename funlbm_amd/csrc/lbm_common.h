@@ -413,27 +413,33 @@ __device__ inline T moving_wall_term(const KParams& p, int face, T rho) {
 // plane rev(K) at the node itself, less the moving-wall term.
 template <class L, class S, int DZ>
 __device__ inline void zface_pull(const KParams& p, const typename S::elem* __restrict__ src, int64_t node,
-                                  const float* __restrict__ rho_prev, typename S::T* f) {
+                                  const float* __restrict__ rho_prev, bool hit, typename S::T* f) {
   using T = typename S::T;
   constexpr int face = DZ > 0 ? 4 : 5;
   const int bc = p.bc[face];
   static_for<0, L::Q>([&](auto kc) {
     constexpr int K = decltype(kc)::value;
     if constexpr (L::E[K][2] == DZ) {
-      const float wr = wt<float>(L::W[K]);
-      if (bc == FUNLBM_BC_PERIODICAL) {
-        const int64_t idx = state_index<L::Q>(p, K, node + (DZ > 0 ? (p.nz - 1) : -(p.nz - 1)));
-        f[K] = S::load(src[idx], wr, p.inv_scale, p.scale);
-      } else {
-        T v = S::load(src[state_index<L::Q>(p, L::REV[K], node)], wr, p.inv_scale, p.scale);
-        if (bc == FUNLBM_BC_WALL_WITH_SPEED) {
-          if constexpr (S::dev)
-            v = v - moving_wall_term<L, K, T>(p, face, (T)rho_prev[node]) * p.scale;
-          else
-            v = v - moving_wall_term<L, K, T>(p, face, (T)rho_prev[node]);
+      // the store is unconditional and only the load is guarded: a guarded
+      // store into the array lets the compiler merge the two faces' stores
+      // through a pointer select, which puts the array in private memory
+      T v = f[K];
+      if (hit) {
+        const float wr = wt<float>(L::W[K]);
+        if (bc == FUNLBM_BC_PERIODICAL) {
+          const int64_t idx = state_index<L::Q>(p, K, node + (DZ > 0 ? (p.nz - 1) : -(p.nz - 1)));
+          v = S::load(src[idx], wr, p.inv_scale, p.scale);
+        } else {
+          v = S::load(src[state_index<L::Q>(p, L::REV[K], node)], wr, p.inv_scale, p.scale);
+          if (bc == FUNLBM_BC_WALL_WITH_SPEED) {
+            if constexpr (S::dev)
+              v = v - moving_wall_term<L, K, T>(p, face, (T)rho_prev[node]) * p.scale;
+            else
+              v = v - moving_wall_term<L, K, T>(p, face, (T)rho_prev[node]);
+          }
         }
-        f[K] = v;
       }
+      f[K] = v;
     }
   });
 }
@@ -504,8 +510,8 @@ __device__ inline void pull_node(const KParams& p, const typename S::elem* __res
                        p.scale);
       });
     }
-    if (c.z == 0) zface_pull<L, S, 1>(p, src, node, rho_prev, f);
-    if (c.z == p.nz - 1) zface_pull<L, S, -1>(p, src, node, rho_prev, f);
+    zface_pull<L, S, 1>(p, src, node, rho_prev, c.z == 0, f);
+    zface_pull<L, S, -1>(p, src, node, rho_prev, c.z == p.nz - 1, f);
     return;
   }
   static_for<0, Q>([&](auto kc) {
