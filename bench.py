@@ -248,12 +248,18 @@ def bench_batch(n_cases=256, steps=3000, seed=0, storage="f16", tick=50, baselin
     one-case-at-a-time solves of the same cases with LBMD3."""
     import torch
 
-    from funlbm_amd.batch.engine import BatchEngine, markers_for
+    from funlbm_amd.batch.engine import BatchEngine, markers_for, prepare_markers
     from funlbm_amd.batch.workloads import settling_batch
     from funlbm_amd.lbm import LBMD3, Config
 
     specs = settling_batch(n_cases=n_cases, seed=seed, steps=steps)
     nodes = sum(s.n for s in specs)
+    # every case's particle shapes, generated in one launch (timed cold)
+    all_particles = [p for s in specs for p in s.particles]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    prepare_markers(all_particles)
+    marker_ms = (time.perf_counter() - t0) * 1e3
     markers = sum(len(markers_for(p)[0]) for s in specs for p in s.particles)
     eng = BatchEngine(storage=storage, max_nodes=nodes + 256 * n_cases, max_markers=markers + 64,
                       max_particles=n_cases, max_cases=n_cases, max_q=19, traj_cap=tick)
@@ -307,6 +313,8 @@ def bench_batch(n_cases=256, steps=3000, seed=0, storage="f16", tick=50, baselin
         "cases": n_cases,
         "nodes": nodes,
         "markers": markers,
+        "marker_generation_ms": round(marker_ms, 2),
+        "marker_shapes": len({(p.kind, p.r, tuple(p.abc or ())) for p in all_particles}),
         "seconds": round(t, 3),
         "wall_seconds": round(wall, 3),
         "solves_per_s": round(solves, 2),
@@ -324,11 +332,12 @@ def bench_serve(rate=20.0, duration=10.0, seed=0, steps=(1000, 20000), budget=20
     """Poisson stream of settling requests served with continuous batching."""
     import torch
 
-    from funlbm_amd.batch.engine import BatchEngine, markers_for
+    from funlbm_amd.batch.engine import BatchEngine, markers_for, prepare_markers
     from funlbm_amd.batch.workloads import serving_requests
     from funlbm_amd.serve import Request, Server
 
     reqs = serving_requests(rate, duration, seed=seed, steps=steps)
+    prepare_markers([p for _, s in reqs for p in s.particles])
     need_nodes = sum(s.n for _, s in reqs)
     pool_nodes = pool_nodes or min(need_nodes + 256 * len(reqs), 1 << 30)
     markers = sum(len(markers_for(p)[0]) for _, s in reqs for p in s.particles)

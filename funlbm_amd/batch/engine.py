@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from .. import _native as nat
-from ..particle.markers import ellipsoid_markers
+from ..particle.markers import surface_markers
 from .alloc import RangeAllocator
 
 TILE = 256
@@ -60,20 +60,29 @@ class CaseSpec:
         return int(np.prod(self.nodes))
 
 
-_marker_cache = {}
+_const_cache = {}
+
+
+def _semi(p: ParticleSpec):
+    if p.kind == "sphere":
+        return (float(p.r),) * 3
+    return tuple(float(v) for v in p.abc)
+
+
+def prepare_markers(particles) -> None:
+    """Generates the markers of every shape among `particles` not seen yet in
+    one GPU launch (admission then finds them cached)."""
+    surface_markers([_semi(p) for p in particles])
 
 
 def markers_for(p: ParticleSpec):
     """Body-frame markers (float32) and the constants of a particle,
     following Sphere/Ellipsoid._init (src/funlbm/particle/base/sphere.py:15-53,
     ellipsoid.py:107-143)."""
-    if p.kind == "sphere":
-        a = b = c = float(p.r)
-    else:
-        a, b, c = (float(v) for v in p.abc)
+    a, b, c = _semi(p)
     key = (p.kind, a, b, c)
-    if key not in _marker_cache:
-        lag = ellipsoid_markers(a, b, c).astype(np.float32)
+    if key not in _const_cache:
+        lag = surface_markers([(a, b, c)])[0]
         mass = float(np.float32(4.0 / 3.0 * math.pi * a * b * c))
         if p.kind == "sphere":
             area = float(np.float32(4.0 * math.pi * a * a))
@@ -82,8 +91,8 @@ def markers_for(p: ParticleSpec):
             area = float(np.float32(4.0 / 3.0 * math.pi * math.pow(a * b * c, 2.0 / 3.0)))
             inertia = np.array([b * c, a * c, a * b]) * np.float32(mass) / 5.0
         lm = np.float32(np.float32(area) / np.float32(len(lag)))
-        _marker_cache[key] = (lag, mass, inertia.astype(np.float32).astype(np.float64), lm)
-    return _marker_cache[key]
+        _const_cache[key] = (lag, mass, inertia.astype(np.float32).astype(np.float64), lm)
+    return _const_cache[key]
 
 
 def lattice_for(spec: CaseSpec, storage: int, dev_scale: float) -> nat.Lattice:

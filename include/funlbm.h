@@ -219,6 +219,21 @@ int funlbm_particle_forces(int32_t n_particles, const int32_t* marker_offsets, c
 int funlbm_particle_pose(int32_t n_particles, const int32_t* marker_offsets, const float* lag,
                          float* lx, double* pstate, int32_t advance, void* stream);
 
+/* Body-frame surface markers of n_shapes ellipsoids, replaces
+ * generate_uniform_points_on_ellipsoid / find_intersection
+ * (src/funlbm/particle/base/ellipsoid.py:10-95) as called by Sphere._init and
+ * Ellipsoid._init (src/funlbm/particle/base/sphere.py:15-53,
+ * ellipsoid.py:107-143).  semi: device double[n_shapes][3] semi-axes (a = b = c
+ * for a sphere).  markers: device float[n_shapes][cap][3]; the first
+ * min(count, cap) rows of each shape are written, in cube order.  counts:
+ * device int32[n_shapes], the full marker count of each shape (a count above
+ * cap asks for a larger cap), or -1 for a shape with a non-positive or
+ * non-finite semi-axis or more than 1024 axis points.  The intersections are
+ * computed in closed form in double (the reference runs fsolve from t = 0.5;
+ * both give the single in-segment root). */
+int funlbm_surface_markers(int32_t n_shapes, const double* semi, double dx, int32_t cap,
+                           float* markers, int32_t* counts, void* stream);
+
 /* Spreading, replaces LBMD3.lagrange_to_flow (src/funlbm/lbm/lbm3d.py:102-119):
  * fol := 0 and then, for every marker in order, fol[stencil] = w lF lm
  * overwriting.  The buffer must be zero outside the stencils whose origins

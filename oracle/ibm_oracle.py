@@ -5,7 +5,8 @@ bench.py's cpu_baseline leg may import it, as the checker.
 Restates, with per-marker Python loops like the reference:
 
   marker generator     src/funlbm/particle/base/ellipsoid.py:10-95 (scipy
-                       fsolve per vertex pair, as the reference)
+                       fsolve per vertex pair, as the reference; and the
+                       closed-form root, vectorised, pinned against it)
   particle constants   src/funlbm/particle/base/sphere.py:15-53,
                        ellipsoid.py:107-143, base.py:108-118
   stencil / weights    src/funlbm/lbm/lbm3d.py:32-64
@@ -72,6 +73,79 @@ def surface_points(a, b, c, dx=1.0):
                     if len(inter) >= 3:
                         pts.append(np.mean(inter, axis=0))
     return np.array(pts)
+
+
+# vertex order of a cube (src/funlbm/particle/base/ellipsoid.py:55-64)
+_CUBE_VERTS = np.array(
+    [[0, 0, 0], [0, 0, 1], [0, 1, 0], [0, 1, 1], [1, 0, 0], [1, 0, 1], [1, 1, 0], [1, 1, 1]],
+    dtype=np.int64,
+)
+_PAIRS = [(i, j) for i in range(8) for j in range(i + 1, 8)]
+
+
+def _segment_root(p1, p2, inv2):
+    """t in [0, 1] with value(p1 + t (p2 - p1)) = 0 for value = sum x^2 inv2 - 1;
+    p1, p2 [n, 3] on opposite sides."""
+    d = p2 - p1
+    A = np.sum(d * d * inv2, axis=1)
+    B = 2.0 * np.sum(p1 * d * inv2, axis=1)
+    C = np.sum(p1 * p1 * inv2, axis=1) - 1.0
+    disc = np.sqrt(np.maximum(B * B - 4.0 * A * C, 0.0))
+    # numerically stable pair of roots
+    qv = -0.5 * (B + np.copysign(disc, B))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r1 = qv / A
+        r2 = C / qv
+    # exactly one root lies in [0, 1]; rounding can push it just outside, so
+    # take the root nearer to the interval
+    d1 = np.maximum(np.maximum(-r1, r1 - 1.0), 0.0)
+    d2 = np.maximum(np.maximum(-r2, r2 - 1.0), 0.0)
+    d1 = np.where(np.isfinite(r1), d1, np.inf)
+    d2 = np.where(np.isfinite(r2), d2, np.inf)
+    return np.where(d1 <= d2, r1, r2)
+
+
+def surface_points_closed_form(a, b, c, dx=1.0):
+    """surface_points with each segment root in closed form, vectorised over
+    the cubes: markers [M, 3] (float64) in the same order.  On a segment whose
+    end points lie on opposite sides of the convex quadric the quadratic in t
+    has exactly one root in [0, 1], the one fsolve from t = 0.5 converges to."""
+    semi = np.array([a, b, c], dtype=np.float64)
+    inv2 = 1.0 / (semi * semi)
+    axes = [np.arange(-s - 2 * dx, s + 2 * dx, dx) for s in (a, b, c)]
+    nx, ny, nz = (len(ax) for ax in axes)
+    X, Y, Z = np.meshgrid(*axes, indexing="ij")
+    val = X * X / (a * a) + Y * Y / (b * b) + Z * Z / (c * c) - 1
+    inside = val < 0
+    # cubes (i, j, k) with i < nx-1 ...
+    ci, cj, ck = np.meshgrid(np.arange(nx - 1), np.arange(ny - 1), np.arange(nz - 1), indexing="ij")
+    ci, cj, ck = ci.ravel(), cj.ravel(), ck.ravel()
+    ins = np.stack(
+        [inside[ci + v[0], cj + v[1], ck + v[2]] for v in _CUBE_VERTS], axis=1
+    )  # [ncubes, 8]
+    cut = ins.any(axis=1) & ~ins.all(axis=1)
+    ci, cj, ck, ins = ci[cut], cj[cut], ck[cut], ins[cut]
+    verts = np.stack(
+        [
+            np.stack([axes[0][ci + v[0]], axes[1][cj + v[1]], axes[2][ck + v[2]]], axis=1)
+            for v in _CUBE_VERTS
+        ],
+        axis=1,
+    )  # [ncut, 8, 3]
+    total = np.zeros((len(ci), 3))
+    count = np.zeros(len(ci), dtype=np.int64)
+    for i, j in _PAIRS:
+        diff = ins[:, i] != ins[:, j]
+        if not diff.any():
+            continue
+        p1 = verts[diff, i]
+        p2 = verts[diff, j]
+        t = _segment_root(p1, p2, inv2)
+        pt = p1 + t[:, None] * (p2 - p1)
+        total[diff] = total[diff] + pt
+        count[diff] += 1
+    keep = count >= 3
+    return total[keep] / count[keep][:, None]
 
 
 class Particle:

@@ -53,7 +53,7 @@ def test_argument_errors_need_no_device():
         nat.check(rc, "state_bytes")
     lat.q = 19
     for i in range(6):
-        lat.bc[i] = 1400  # NON_EQUILIBRIUM is not implemented by the HIP path
+        lat.bc[i] = 1400  # NON_EQUILIBRIUM: the reference defines it on the x faces only
     rc = nat.lib().funlbm_state_bytes(ctypes.byref(lat), ctypes.byref(ctypes.c_int64()))
     assert rc == nat.EUNSUPPORTED_BC
     with pytest.raises(NotImplementedError):
@@ -64,6 +64,32 @@ def test_argument_errors_need_no_device():
     assert nat.lib().funlbm_state_bytes(ctypes.byref(lat), ctypes.byref(nbytes)) == 0
     # one 256-node tile of 19 planes, plus a slack tile on either side
     assert nbytes.value == 19 * 3 * 256 * 4
+
+
+def test_marker_generator_argument_errors_need_no_device():
+    L = nat.lib()
+    p = ctypes.c_void_p(16)  # never dereferenced: each call fails its host checks
+    assert L.funlbm_surface_markers(0, None, 1.0, 0, None, None, None) == 0
+    assert L.funlbm_surface_markers(-1, p, 1.0, 8, p, p, None) == nat.EINVAL
+    assert L.funlbm_surface_markers(1, p, 1.0, -1, p, p, None) == nat.EINVAL
+    assert L.funlbm_surface_markers(1, p, 0.0, 8, p, p, None) == nat.EINVAL
+    assert L.funlbm_surface_markers(1, p, float("nan"), 8, p, p, None) == nat.EINVAL
+    assert L.funlbm_surface_markers(1, None, 1.0, 8, p, p, None) == nat.EINVAL
+    assert L.funlbm_surface_markers(1, p, 1.0, 8, None, p, None) == nat.EINVAL
+    assert "NULL" in nat.last_error()
+
+
+def test_marker_wrapper_rejects_bad_shapes_before_any_device_call():
+    from funlbm_amd.particle.markers import surface_markers
+
+    with pytest.raises(ValueError):
+        surface_markers([(1.0, -2.0, 1.0)])
+    with pytest.raises(ValueError):
+        surface_markers([(1.0, 1.0, float("inf"))])
+    with pytest.raises(ValueError):
+        surface_markers([(1.0, 1.0, 1.0)], dx=0.0)
+    with pytest.raises(RuntimeError):
+        surface_markers([(1.0, 1.0, 1.0)], device="cpu")
 
 
 @pytest.mark.parametrize("dims,storage", [((4, 4, 4), 0), ((7, 9, 11), 1), ((64, 64, 64), 0), ((256, 256, 256), 1),

@@ -1,5 +1,5 @@
-"""Pins of the oracle's immersed-boundary and particle functions (CPU), and
-of the host-side marker generator against the oracle's fsolve generator."""
+"""Pins of the oracle's immersed-boundary and particle functions (CPU),
+among them the closed-form marker generator against the fsolve generator."""
 
 import math
 
@@ -11,23 +11,20 @@ from oracle import lbm_oracle as orc
 
 
 # ---------------------------------------------------------------- markers
-@pytest.mark.parametrize("abc", [(3, 3, 3), (4.5, 4.5, 4.5), (3, 3, 6), (2.5, 3.5, 4)])
-def test_host_marker_generator_matches_fsolve_generator(abc):
-    """Closed-form in-segment roots (product) vs scipy fsolve from t = 0.5
-    (the reference's method): same markers, same order."""
-    from funlbm_amd.particle.markers import ellipsoid_markers
-
-    got = ellipsoid_markers(*abc)
-    want = ibo.surface_points(*abc)
+@pytest.mark.parametrize("abc,dx", [((3, 3, 3), 1.0), ((4.5, 4.5, 4.5), 1.0), ((3, 3, 6), 1.0),
+                                    ((2.5, 3.5, 4), 1.0), ((3.07, 3.07, 3.07), 0.7), ((1.3, 2.9, 2.1), 0.45)])
+def test_closed_form_marker_generator_matches_fsolve_generator(abc, dx):
+    """Closed-form in-segment roots vs scipy fsolve from t = 0.5 (the
+    reference's method): same markers, same order."""
+    got = ibo.surface_points_closed_form(*abc, dx=dx)
+    want = ibo.surface_points(*abc, dx=dx)
     assert got.shape == want.shape
     assert np.max(np.abs(got - want)) < 1e-7
 
 
 def test_markers_lie_near_the_surface_and_cover_it():
-    from funlbm_amd.particle.markers import ellipsoid_markers
-
     for a, b, c in [(6, 6, 6), (4, 4, 8)]:
-        m = ellipsoid_markers(a, b, c)
+        m = ibo.surface_points_closed_form(a, b, c)
         val = (m[:, 0] / a) ** 2 + (m[:, 1] / b) ** 2 + (m[:, 2] / c) ** 2
         # centroids of the cut polygon sit within one cell of the surface
         assert np.all(np.abs(np.sqrt(val) - 1) < 1.0 / min(a, b, c))
