@@ -26,7 +26,7 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "MLUPS & %HBM roofline (D3Q19 256³); batched solves/s + p99 at fixed error"
 
 
-def build_solver(n, q, storage, tau=0.6, seed=0):
+def build_solver(n, q, storage, tau=0.6, seed=0, rounding=None):
     import torch
 
     from funlbm_amd import _native as nat
@@ -42,6 +42,7 @@ def build_solver(n, q, storage, tau=0.6, seed=0):
                 "param": {"mu": mu},
                 "boundary": faces,
                 "storage": storage,
+                **({"rounding": rounding} if rounding else {}),
             },
             "log_every": 0,
         }
@@ -128,11 +129,11 @@ def profiled_traffic(storage, size, q):
     return None
 
 
-def storage_leg(size, q, storage, steps, warmup, ref_mlups=None):
+def storage_leg(size, q, storage, steps, warmup, ref_mlups=None, rounding=None):
     """Step and kernel timing of one lattice/storage at size^3 (periodic, no particles)."""
     import torch
 
-    lbm = build_solver(size, q, storage)
+    lbm = build_solver(size, q, storage, rounding=rounding)
     ms, _ = time_steps(lbm, steps, max(warmup, 50))
     kms = time_fused_kernel(lbm)
     n = lbm.flow.n
@@ -141,7 +142,8 @@ def storage_leg(size, q, storage, steps, warmup, ref_mlups=None):
     elem = {"f32": 4, "f16": 2, "bf16": 2, "f64": 8}[storage]
     mlups = n / (ms * 1e-3) / 1e6
     ach = n * 2 * q * elem / (kms * 1e-3) / 1e9
-    out = {"workload": f"D3Q{q} {size}^3 periodic, tau 0.6, {storage} state", "mlups": round(mlups, 1),
+    rd = f", {rounding} rounding" if rounding else ""
+    out = {"workload": f"D3Q{q} {size}^3 periodic, tau 0.6, {storage} state{rd}", "mlups": round(mlups, 1),
            "ms_per_step": round(ms, 4), "kernel_ms": round(kms, 4), "bytes_per_lu": 2 * q * elem,
            "achieved_gbs": round(ach, 1), "frac_of_8tbs": round(ach / HBM_PEAK_GBS, 4)}
     if ref_mlups:
@@ -479,12 +481,19 @@ def main():
             "kernel_ms": round(k16, 4),
             "achieved_gbs": round(n * 2 * args.q * 2 / (k16 * 1e-3) / 1e9, 1),
         }
+        out["f16_deviation"]["rounding"] = "stochastic"
+        out["f16_nearest"] = storage_leg(args.size, args.q, "f16", args.steps, args.warmup, mlups,
+                                         rounding="nearest")
     if not args.no_extra and args.storage == "f32":
         try:
             out["bf16_deviation"] = storage_leg(args.size, args.q, "bf16", args.steps, args.warmup, mlups)
             q27 = storage_leg(args.size, 27, "f32", args.steps, args.warmup)
+            out["bf16_nearest"] = storage_leg(args.size, args.q, "bf16", args.steps, args.warmup, mlups,
+                                              rounding="nearest")
             out["d3q27"] = {"f32": q27,
-                            "f16": storage_leg(args.size, 27, "f16", args.steps, args.warmup, q27["mlups"])}
+                            "f16": storage_leg(args.size, 27, "f16", args.steps, args.warmup, q27["mlups"]),
+                            "f16_nearest": storage_leg(args.size, 27, "f16", args.steps, args.warmup,
+                                                       q27["mlups"], rounding="nearest")}
             out["suspension"] = bench_suspension(size=args.size)
         except Exception as e:  # reported, never silent
             out["extra_error"] = repr(e)

@@ -22,6 +22,12 @@ BC_WALL = 1200
 BC_WALL_WITH_SPEED = 1201
 BC_NON_EQUILIBRIUM = 1400
 
+ABI_VERSION = 2
+
+ROUND_NEAREST = 0
+ROUND_STOCHASTIC = 1
+ROUNDING_CODES = {"nearest": ROUND_NEAREST, "stochastic": ROUND_STOCHASTIC}
+
 STORE_F32 = 0
 STORE_F16 = 1
 STORE_BF16 = 2
@@ -63,6 +69,8 @@ class Lattice(ctypes.Structure):
         ("dev_scale", c_float),
         ("body_force", c_float * 3),
         ("uw", (c_float * 3) * 6),
+        ("rounding", c_int32),
+        ("seed", ctypes.c_uint32),
     ]
 
 
@@ -125,7 +133,7 @@ def lib() -> ctypes.CDLL:
                 fn = getattr(h, name)
                 fn.restype = res
                 fn.argtypes = args
-            if h.funlbm_abi_version() != 1:
+            if h.funlbm_abi_version() != ABI_VERSION:
                 raise RuntimeError("libfunlbm_hip.so has an unexpected ABI version")
             _lib = h
     return _lib
@@ -157,6 +165,12 @@ def ptr(t):
 
 def stream_handle(device=None):
     return c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def step_seed(base: int, steps_done: int) -> int:
+    """The dither seed of the step after `steps_done` steps from `base`
+    (step_seed in the kernels' common header)."""
+    return (int(base) + int(steps_done) * 0x9E3779B9) & 0xFFFFFFFF
 
 
 def state_plane(lat: Lattice) -> int:

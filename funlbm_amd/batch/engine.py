@@ -54,6 +54,8 @@ class CaseSpec:
     gravity: tuple = (0.0, 0.0, 2e-5)
     bcs: tuple = (nat.BC_WALL,) * 6
     uzero_mask: int = 0x3F
+    # base seed of the stochastic rounding's dither (16-bit states)
+    seed: int = 0
 
     @property
     def n(self):
@@ -95,7 +97,7 @@ def markers_for(p: ParticleSpec):
     return _const_cache[key]
 
 
-def lattice_for(spec: CaseSpec, storage: int, dev_scale: float) -> nat.Lattice:
+def lattice_for(spec: CaseSpec, storage: int, dev_scale: float, rounding: int = nat.ROUND_NEAREST) -> nat.Lattice:
     lat = nat.Lattice()
     lat.q = spec.q
     lat.nx, lat.ny, lat.nz = (int(v) for v in spec.nodes)
@@ -106,6 +108,8 @@ def lattice_for(spec: CaseSpec, storage: int, dev_scale: float) -> nat.Lattice:
     lat.tau = float(np.float32(spec.tau))
     lat.rho0 = 1.0
     lat.dev_scale = dev_scale
+    lat.rounding = rounding
+    lat.seed = int(spec.seed) & 0xFFFFFFFF
     return lat
 
 
@@ -129,13 +133,21 @@ class BatchEngine:
     `max_markers` markers and `max_particles` particles in total."""
 
     def __init__(self, storage="f16", max_nodes=1 << 24, max_markers=1 << 16, max_particles=256,
-                 max_cases=256, max_q=27, traj_cap=64, dev_scale=1024.0, device=None):
+                 max_cases=256, max_q=27, traj_cap=64, dev_scale=1024.0, device=None, rounding=None):
         from ..base.work import device_detect
 
         self.device = device_detect(device)
         dev = self.device
         self.storage = nat.STORAGE_CODES[storage]
         self.storage_name = storage
+        # 16-bit states round stochastically unless told otherwise
+        rounding = rounding or ("stochastic" if storage in ("f16", "bf16") else "nearest")
+        if rounding not in nat.ROUNDING_CODES:
+            raise ValueError(f"unknown rounding {rounding!r}")
+        if rounding == "stochastic" and storage not in ("f16", "bf16"):
+            raise ValueError("stochastic rounding applies to f16/bf16 storage only")
+        # cases of a stochastic engine draw their dither from step_seed(spec.seed, steps done)
+        self.rounding = nat.ROUNDING_CODES[rounding]
         self.dev_scale = float(dev_scale)
         self.max_cases = int(max_cases)
         self.traj_cap = int(traj_cap)
@@ -222,7 +234,7 @@ class BatchEngine:
         m_off = self.markers.alloc(m_count) if m_count else 0
         p_off = self.parts.alloc(len(spec.particles)) if spec.particles else 0
         cid = self.free_cases.pop()
-        lat = lattice_for(spec, self.storage, self.dev_scale)
+        lat = lattice_for(spec, self.storage, self.dev_scale, self.rounding)
         desc = (ctypes.c_uint8 * self.case_bytes)()
         nat.call("funlbm_case_pack", lat, f_off, node_off, tile_off, desc)
         self.cases[cid * self.case_bytes : (cid + 1) * self.case_bytes] = torch.frombuffer(

@@ -96,7 +96,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_fused(
     const bool new_here = entry == stamp;
     const bool old_here = entry == stamp || entry == stamp - 1u;
     FusedIO<S> io{src, dst, fo, fn, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    fused_step_node<L, S, true>(cd.p, node, io, new_here, old_here);
+    // the case's dither seed advances with its own step count, as a
+    // single-lattice solve's does with its steps
+    const uint32_t mix = mix32(step_seed(cd.p.seed, (uint32_t)dyn.done[c]));
+    fused_step_node<L, S, true>(cd.p, node, io, new_here, old_here, mix);
   }
 }
 

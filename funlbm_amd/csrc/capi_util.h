@@ -144,6 +144,13 @@ inline int make_params(const funlbm_lattice* lat, KParams* p) {
   if (dev && lat->rho0 != 1.0f)
     return set_error(FUNLBM_EINVAL, "rho0 must be 1 (the deviation base of the 16-bit states is w_k), got %g",
                      (double)lat->rho0);
+  if (lat->rounding != FUNLBM_ROUND_NEAREST && lat->rounding != FUNLBM_ROUND_STOCHASTIC)
+    return set_error(FUNLBM_EINVAL, "unknown rounding code %d", lat->rounding);
+  if (lat->rounding == FUNLBM_ROUND_STOCHASTIC && !dev)
+    return set_error(FUNLBM_EINVAL, "stochastic rounding applies to F16/BF16 storage only");
+  p->sr = lat->rounding == FUNLBM_ROUND_STOCHASTIC;
+  p->seed = lat->seed;
+  p->seed_mix = mix32(lat->seed);
   p->rho0 = lat->rho0;
   p->scale = dev ? lat->dev_scale : 1.0f;
   p->inv_scale = dev ? 1.0f / lat->dev_scale : 1.0f;

@@ -247,7 +247,7 @@ __device__ __forceinline__ void fused_kernel_body(
   if constexpr (MAT)
     fused_node<L, S, FORCE, true>(p, node, io, new_here, old_here);
   else
-    fused_step_node<L, S, FORCE>(p, node, io, new_here, old_here);
+    fused_step_node<L, S, FORCE>(p, node, io, new_here, old_here, p.seed_mix);
 }
 
 #define FUNLBM_FUSED_ARGS                                                                              \

@@ -127,10 +127,11 @@ __device__ __forceinline__ void fused_node(const KParams& p, int64_t node, const
 
 // The fused step of a 16-bit state in deviation arithmetic (moments_dev,
 // add_forcing_dev, collide_dev): the populations stay scaled deviations from
-// the load to the store.
+// the load to the store, which rounds to nearest or, with p.sr, stochastically
+// with the dither of this step's seed mix (mix32 of the step's seed).
 template <class L, class S, bool FORCE>
 __device__ __forceinline__ void fused_node_dev(const KParams& p, int64_t node, const FusedIO<S>& io,
-                                               bool new_here, bool old_here) {
+                                               bool new_here, bool old_here, uint32_t seed_mix) {
   using D = DevOf<S>;
   const Node c = node_coords(p, node);
   float g[L::Q];
@@ -164,20 +165,34 @@ __device__ __forceinline__ void fused_node_dev(const KParams& p, int64_t node, c
   const uint32_t tmask = (1u << ts) - 1u;
   const int tn = (__builtin_amdgcn_readfirstlane((int)node) & ~63) >> ts;
   const uint32_t lb = ((uint32_t)node & tmask) * (uint32_t)sizeof(R);
-  static_for<0, Q>([&](auto kc) {
-    constexpr int K = decltype(kc)::value;
-    char* base = (char*)io.dst + ((int64_t)Q + (int64_t)tn * Q + K) * ((int64_t)1 << ts) * (int64_t)sizeof(R);
-    __builtin_nontemporal_store(S::from_float(g[K]), (R*)(base + lb));
-  });
+  // one straight store sequence per rounding mode (a per-direction select
+  // would split the stores into blocks and cost an address add each)
+  if (p.sr) {
+    const uint32_t h = mix32((uint32_t)node ^ seed_mix);
+    // two conversions per register (low and high half), no register to clear
+    uint32_t pair = 0;
+    static_for<0, Q>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      char* base = (char*)io.dst + ((int64_t)Q + (int64_t)tn * Q + K) * ((int64_t)1 << ts) * (int64_t)sizeof(R);
+      pair = S::template cvt_sr_half<(K & 1) != 0>(pair, g[K], __umul24(h, dither_mult(K)));
+      __builtin_nontemporal_store((R)(K & 1 ? pair >> 16 : pair), (R*)(base + lb));
+    });
+  } else {
+    static_for<0, Q>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      char* base = (char*)io.dst + ((int64_t)Q + (int64_t)tn * Q + K) * ((int64_t)1 << ts) * (int64_t)sizeof(R);
+      __builtin_nontemporal_store(S::from_float(g[K]), (R*)(base + lb));
+    });
+  }
 }
 
 // The fused step of one node: deviation arithmetic for the 16-bit states,
 // the reference's operations otherwise.
 template <class L, class S, bool FORCE>
 __device__ __forceinline__ void fused_step_node(const KParams& p, int64_t node, const FusedIO<S>& io,
-                                                bool new_here, bool old_here) {
+                                                bool new_here, bool old_here, uint32_t seed_mix) {
   if constexpr (S::fast)
-    fused_node_dev<L, S, FORCE>(p, node, io, new_here, old_here);
+    fused_node_dev<L, S, FORCE>(p, node, io, new_here, old_here, seed_mix);
   else
     fused_node<L, S, FORCE, false>(p, node, io, new_here, old_here);
 }

@@ -136,6 +136,14 @@ struct StoreF16 {
   __device__ static inline elem store(float f, float wrho0, float, float scale) {
     return __float2half_rn((f - wrho0) * scale);
   }
+  // stochastic rounding into the low (HI false) or high half of `old`: up by
+  // one ulp iff the discarded bits, truncated to 13, plus rnd >> 19 carry out
+  // of 13 bits (v_cvt_sr_f16_f32)
+  template <bool HI>
+  __device__ static inline uint32_t cvt_sr_half(uint32_t old, float g, uint32_t rnd) {
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(h2, old), g, rnd, HI));
+  }
   // the same conversions on the 16-bit patterns (vector loads and stores)
   using raw = unsigned short;
   __device__ static inline float load_raw(raw v, float wrho0, float inv_scale, float s) {
@@ -162,6 +170,14 @@ struct StoreBF16 {
   __device__ static inline elem store(float f, float wrho0, float, float scale) {
     return __float2bfloat16(((f - wrho0) * scale));
   }
+  // stochastic rounding into the low (HI false) or high half of `old`: up by
+  // one ulp iff the 16 discarded bits plus rnd >> 16 carry out of 16 bits
+  // (v_cvt_sr_bf16_f32)
+  template <bool HI>
+  __device__ static inline uint32_t cvt_sr_half(uint32_t old, float g, uint32_t rnd) {
+    typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(b2, old), g, rnd, HI));
+  }
   using raw = unsigned short;
   __device__ static inline float load_raw(raw v, float wrho0, float inv_scale, float s) {
     return load(__builtin_bit_cast(elem, v), wrho0, inv_scale, s);
@@ -185,6 +201,41 @@ struct DevOf {
   __device__ static inline float load(elem v, float, float, float) { return S::to_float(v); }
   __device__ static inline raw store_raw(float g, float, float, float) { return S::from_float(g); }
 };
+
+// ---------------------------------------------------------------------------
+// Dither of the stochastic rounding.  The 32-bit word of population k at node
+// j in a step of seed s is the low 32 bits of (h mod 2^24) * C_k with
+// h = mix32(j ^ mix32(s)) (one v_mul_u32_u24 per population).  The
+// conversions use the word's top 13 (f16) or 16 (bf16) bits: a multiplicative
+// hash of h per direction.  C_k are odd multipliers in [2^23, 2^24), taken in
+// order from (mix32(i) mod 2^24) | 0x800001, i = 1, 2, ..., skipping any whose
+// top bits correlate with an earlier one's by more than 0.012 over 3 x 2^16
+// nodes (multipliers in a near small-integer ratio, such as C' = 2C - 1, give
+// correlated dithers).
+// mix32 is the 32-bit integer hash x ^= x >> 16; x *= 0x7feb352d;
+// x ^= x >> 15; x *= 0x846ca68b; x ^= x >> 16 (C. Wellons' "lowbias32").
+// ---------------------------------------------------------------------------
+__host__ __device__ inline uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// The seed of the step after `steps_done` steps from base seed `base`.
+__host__ __device__ inline uint32_t step_seed(uint32_t base, uint32_t steps_done) {
+  return base + steps_done * 0x9E3779B9u;
+}
+
+__host__ __device__ constexpr uint32_t dither_mult(int k) {
+  constexpr uint32_t C[27] = {0x8990c1, 0x932181, 0xf1e9dd, 0xfe5ed1, 0xc5d53f, 0xe3d3bb, 0x8ba1e7,
+                              0xd35fbb, 0x9fd5ab, 0x8baa7d, 0xa8f9fb, 0xdc746d, 0xbfcff1, 0xf63447,
+                              0xbd4a6f, 0xa56ae5, 0xabdd59, 0xbb2f91, 0x8001ed, 0xdd7671, 0xbe2e8f,
+                              0xdbcac3, 0x9006fb, 0xce98a9, 0xff9fe3, 0xbc09cb, 0xdc5b09};
+  return C[k];
+}
 
 // ---------------------------------------------------------------------------
 // Kernel-side lattice parameters, built from funlbm_lattice on the host.
@@ -212,6 +263,9 @@ struct KParams {
   int has_moving;  // any WALL_WITH_SPEED face
   int has_neq;     // a NON_EQUILIBRIUM input/output face (explicit path only)
   int has_body;    // body_force != 0
+  int sr;          // stochastic rounding of a 16-bit state's stores
+  uint32_t seed;   // the lattice's dither seed (the base, for a batched case)
+  uint32_t seed_mix;  // dither_mix(seed): this step's mix (single-lattice calls)
   float tau;       // float32 tau as the reference's field
   float rtau;      // float32(1 / tau)
   float omega;     // 1 - 1/(2 tau) in float32, the forcing prefactor
@@ -497,9 +551,13 @@ __device__ inline void pull_node(const KParams& p, const typename S::elem* __res
           const uint32_t rkb = (uint32_t)(-so) & tbmask;
           const char* ua = tb + so;
           const char* sb = rkb <= l0b ? ua : ua - back;
-          const uint32_t vo = (rkb > l0b) & (rkb < l0b + 64u * (uint32_t)sizeof(E)) ? lb + (lb >= rkb ? back : 0u) : lb;
+          const uint32_t vo = ((rkb > l0b) & (rkb < l0b + 64u * (uint32_t)sizeof(E))) ? lb + (lb >= rkb ? back : 0u) : lb;
           f[K] = S::load(*(const E*)(sb + vo), wt<float>(L::W[K]), p.inv_scale, p.scale);
         });
+        // keeps the compiler from sinking both paths' loads into one block
+        // behind per-lane 64-bit addresses: the common path above then loads
+        // from a scalar base plus the lane's 32-bit offset
+        __asm__ volatile("" ::: "memory");
       }
     } else {
       static_for<0, Q>([&](auto kc) {
@@ -580,6 +638,31 @@ __device__ inline T edotu(const T u[3]) {
   if (L::E[K][1] == -1) d = d - u[1];
   if (L::E[K][2] == 1) d = d + u[2];
   if (L::E[K][2] == -1) d = d - u[2];
+  return d;
+}
+
+// The first direction with a nonzero component along axis d.
+template <class L>
+__host__ __device__ constexpr int first_along(int d) {
+  for (int k = 0; k < L::Q; ++k)
+    if (L::E[k][d] != 0) return k;
+  return 0;
+}
+
+// e_k . u without the leading zero of edotu (the deviation path, where the
+// sign of a zero product does not matter): one add or subtract fewer.
+template <class L, int K>
+__device__ inline float edotu_dev(const float u[3]) {
+  constexpr int a = L::E[K][0], b = L::E[K][1], c = L::E[K][2];
+  float d = a == 1 ? u[0] : a == -1 ? -u[0] : b == 1 ? u[1] : b == -1 ? -u[1] : c == 1 ? u[2] : -u[2];
+  if constexpr (a != 0) {
+    if constexpr (b == 1) d = d + u[1];
+    if constexpr (b == -1) d = d - u[1];
+  }
+  if constexpr (a != 0 || b != 0) {
+    if constexpr (c == 1) d = d + u[2];
+    if constexpr (c == -1) d = d - u[2];
+  }
   return d;
 }
 
@@ -702,17 +785,22 @@ __device__ inline void add_forcing(T* f, const T u[3], const T F[3], const Relax
 template <class L>
 __device__ inline void moments_dev(const KParams& p, const Node& c, const float* g, const float F[3], float& rho,
                                    float u[3]) {
-  float r = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
-#pragma unroll
-  for (int k = 0; k < L::Q; ++k) {
+  // each sum starts from its first term (no leading zero to add)
+  constexpr int fx = first_along<L>(0), fy = first_along<L>(1), fz = first_along<L>(2);
+  float r = g[0];
+  float jx = L::E[fx][0] == 1 ? g[fx] : -g[fx];
+  float jy = L::E[fy][1] == 1 ? g[fy] : -g[fy];
+  float jz = L::E[fz][2] == 1 ? g[fz] : -g[fz];
+  static_for<1, L::Q>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
     r = r + g[k];
-    if (L::E[k][0] == 1) jx = jx + g[k];
-    if (L::E[k][0] == -1) jx = jx - g[k];
-    if (L::E[k][1] == 1) jy = jy + g[k];
-    if (L::E[k][1] == -1) jy = jy - g[k];
-    if (L::E[k][2] == 1) jz = jz + g[k];
-    if (L::E[k][2] == -1) jz = jz - g[k];
-  }
+    if constexpr (k != fx && L::E[k][0] == 1) jx = jx + g[k];
+    if constexpr (k != fx && L::E[k][0] == -1) jx = jx - g[k];
+    if constexpr (k != fy && L::E[k][1] == 1) jy = jy + g[k];
+    if constexpr (k != fy && L::E[k][1] == -1) jy = jy - g[k];
+    if constexpr (k != fz && L::E[k][2] == 1) jz = jz + g[k];
+    if constexpr (k != fz && L::E[k][2] == -1) jz = jz - g[k];
+  });
   rho = __builtin_fmaf(r, p.inv_scale, 1.0f);
   const float rr = __builtin_amdgcn_rcpf(rho);
   u[0] = __builtin_fmaf(F[0], 0.5f, jx * p.inv_scale) * rr;
@@ -725,28 +813,30 @@ __device__ inline void moments_dev(const KParams& p, const Node& c, const float*
   }
 }
 
+// g <- g (1 - 1/tau) + (s/tau) w_k (rho a_k - 1), the BGK relaxation towards
+// g_eq = s w_k (rho a_k - 1), a_k = 1 + t + t^2/2 - 3|u|^2/2, t = 3 e_k.u.
+// With c0 = rho (1 - 3|u|^2/2) - 1, rho a_k - 1 = c0 + rho t (1 + t/2).  A
+// direction and its reverse (t -> -t) share, with sw = (s/tau) w_k,
+// anti = sw rho t = (3 sw rho)(e_k.u) and sym = sw c0 + anti (3/2)(e_k.u), and
+// take sym +- anti; 3 sw rho and sw c0 are formed once per weight.
 template <class L>
 __device__ inline void collide_dev(const KParams& p, float* g, const float u[3], float rho) {
   const float uu = __builtin_fmaf(u[2], u[2], __builtin_fmaf(u[1], u[1], u[0] * u[0]));
-  const float u2 = uu * 1.5f;
+  const float c0 = __builtin_fmaf(rho, __builtin_fmaf(uu, -1.5f, 1.0f), -1.0f);
   const float sr = p.scale * p.rtau;  // s / tau
-  // a direction and its reverse together: t_rev = -t, so no table of t is kept
-  auto relax = [&](auto kc, float t) {
-    constexpr int K = decltype(kc)::value;
-    const float a = __builtin_fmaf(t, __builtin_fmaf(t, 0.5f, 1.0f), 1.0f) - u2;
-    // g += (g_eq - g) / tau, g_eq = s w_k (rho a - 1)
-    const float geq_tau = (sr * wt<float>(L::W[K])) * __builtin_fmaf(rho, a, -1.0f);
-    g[K] = __builtin_fmaf(-g[K], p.rtau, g[K]) + geq_tau;
-  };
+  const float keep = 1.0f - p.rtau;
   static_for<0, L::Q>([&](auto kc) {
     constexpr int K = decltype(kc)::value;
     constexpr int R = L::REV[K];
+    const float sw = sr * wt<float>(L::W[K]);
     if constexpr (R == K) {
-      relax(kc, 0.0f);
+      g[K] = __builtin_fmaf(g[K], keep, sw * c0);
     } else if constexpr (R > K) {
-      const float t = edotu<L, K, float>(u) * 3.0f;
-      relax(kc, t);
-      relax(std::integral_constant<int, R>{}, -t);
+      const float eu = edotu_dev<L, K>(u);
+      const float anti = ((3.0f * sw) * rho) * eu;
+      const float sym = __builtin_fmaf(anti, 1.5f * eu, sw * c0);
+      g[K] = __builtin_fmaf(g[K], keep, sym + anti);
+      g[R] = __builtin_fmaf(g[R], keep, sym - anti);
     }
   });
 }

@@ -100,6 +100,12 @@ class FlowConfig(BaseConfig):
       storage      "f32" (default), "f16", "bf16" or "f64": population storage
                    of the fused timestep (f16/bf16 keep f - w rho0, scaled)
       dev_scale    scale of the f16/bf16 deviation storage (default 1024)
+      rounding     "stochastic" (default for f16/bf16) or "nearest": rounding
+                   of the f16/bf16 state at each step's store (stochastic:
+                   unbiased, dithered by a hash of seed, step, node and
+                   direction, so sub-ulp increments such as a weak body
+                   force accumulate); float32/float64 states round to nearest
+      seed         base seed of the stochastic rounding's dither (default 0)
       uzero_faces  list of face names on which u is zeroed after each moment
                    update (default: all six, the reference's behaviour)
       body_force   [gx, gy, gz] added to FOL everywhere (default none)
@@ -120,6 +126,8 @@ class FlowConfig(BaseConfig):
         self.mu: float = 10
         self.storage: str = "f32"
         self.dev_scale: float = 1024.0
+        self.rounding = None
+        self.seed: int = 0
         self.uzero_faces = None
         self.body_force = None
         self.gravity = None
@@ -135,6 +143,8 @@ class FlowConfig(BaseConfig):
         self.gl = float(deep_get(self.param, "gl") or self.gl)
         self.storage = str(deep_get(config_json, "storage") or self.storage)
         self.dev_scale = float(deep_get(config_json, "dev_scale") or self.dev_scale)
+        self.rounding = deep_get(config_json, "rounding") or self.rounding
+        self.seed = int(deep_get(config_json, "seed") or self.seed)
         self.uzero_faces = deep_get(config_json, "uzero_faces")
         self.body_force = deep_get(config_json, "body_force")
         self.gravity = deep_get(self.param, "gravity") or deep_get(config_json, "gravity")

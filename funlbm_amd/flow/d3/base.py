@@ -58,6 +58,16 @@ class FlowD3(Flow):
             raise ValueError(f"unknown storage {storage!r}; expected one of {list(nat.STORAGE_CODES)}")
         self.storage = storage
         self.dev_scale = float(getattr(config, "dev_scale", 1024.0) or 1024.0)
+        rounding = getattr(config, "rounding", None) or ("stochastic" if storage in ("f16", "bf16") else "nearest")
+        if rounding not in nat.ROUNDING_CODES:
+            raise ValueError(f"unknown rounding {rounding!r}; expected one of {list(nat.ROUNDING_CODES)}")
+        if rounding == "stochastic" and storage not in ("f16", "bf16"):
+            raise ValueError("stochastic rounding applies to f16/bf16 storage only")
+        self.rounding = rounding
+        self.seed = int(getattr(config, "seed", 0) or 0) & 0xFFFFFFFF
+        # fused steps taken since init: the stochastic rounding's dither of a
+        # step is drawn from step_seed(seed, steps_done)
+        self.steps_done = 0
         self.uzero_mask = _uzero_mask(getattr(config, "uzero_faces", None))
         bf = getattr(config, "body_force", None)
         self.body_force = None if bf is None else [float(v) for v in bf]
@@ -81,6 +91,9 @@ class FlowD3(Flow):
         lat.tau = float(np.float32(3 * self.config.mu + 0.5))
         lat.rho0 = 1.0
         lat.dev_scale = self.dev_scale
+        if storage_code == nat.STORAGE_CODES[self.storage]:
+            lat.rounding = nat.ROUNDING_CODES[self.rounding]
+        lat.seed = self.seed
         if self.body_force is not None:
             for d in range(3):
                 lat.body_force[d] = self.body_force[d]
@@ -292,12 +305,14 @@ class FlowD3(Flow):
             and (fol_old is None or self._fol_tiled[self._fc])
         )
         self._tiles = (tiles, ctl) if use_tiles else (None, None)
+        self.lat.seed = nat.step_seed(self.seed, self.steps_done)
         nat.call(
             "funlbm_fused_step", self.lat, nat.ptr(self._state[a]), nat.ptr(self._state[b]),
             fol_old, fol_new, nat.ptr(self._tiles[0]), nat.ptr(self._tiles[1]),
             nat.ptr(self._rho_face[a]), nat.ptr(self._rho_face[b]), self._stream(),
         )
         self._cur = b
+        self.steps_done += 1
         self.publish_fol(fol_new_written, tiled=use_tiles)
         self._mat_valid = False
 

@@ -8,9 +8,10 @@ lrou}`.  This package writes the same datasets, under the same paths as keys,
 into a compressed NumPy archive (h5py is not a dependency), plus
 `particle/particle_{i:06d}/state`: the particle's float64 state row
 (centre, velocities, force, torque, rotation vector and matrix, mass, inertia,
-density, gravity), which restores its rotation exactly, and `flow/FOL`, the
+density, gravity), which restores its rotation exactly, `flow/FOL`, the
 force density of the last spreading, which the next step's first moment
-update reads.  Loading sets FOL (zero if the archive has none, as the
+update reads, and `flow/steps_done`, the fused steps taken, from which the
+stochastic rounding of a 16-bit state draws its next dither.  Loading sets FOL (zero if the archive has none, as the
 reference's load leaves it), then f, recomputes the moments and restores the
 particle datasets, so a resumed run equals the uninterrupted one.
 """
@@ -33,6 +34,8 @@ def dump(lbm, path: str) -> str:
             "flow/f": lbm.flow.f.detach().cpu().numpy()}
     if hasattr(lbm.flow, "fol_array"):
         data["flow/FOL"] = lbm.flow.fol_array().detach().cpu().numpy()
+    if hasattr(lbm.flow, "steps_done"):
+        data["flow/steps_done"] = np.array([lbm.flow.steps_done], dtype=np.int64)
     sw = lbm.particle_swarm
     for i, p in enumerate(sw.particles):
         g = _particle_group(i)
@@ -57,6 +60,8 @@ def load(lbm, path: str) -> None:
             fl.FOL = torch.from_numpy(np.ascontiguousarray(z["flow/FOL"]))
         fl.f = torch.from_numpy(np.ascontiguousarray(f))
         fl.update_u_rou()
+        if "flow/steps_done" in z.files and hasattr(fl, "steps_done"):
+            fl.steps_done = int(z["flow/steps_done"][0])
         sw = lbm.particle_swarm
         for i, p in enumerate(sw.particles):
             g = _particle_group(i)

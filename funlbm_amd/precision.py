@@ -2,9 +2,10 @@
 
 Each study runs one configuration through LBMD3 once per storage format
 (float64 populations and arithmetic as the reference solution, then float32,
-f16 and bf16 deviation storage with float32 arithmetic) and reports, per
-format, the relative L-inf error of the velocity field and the largest
-particle-centroid distance from the float64 run.
+f16 and bf16 deviation storage with float32 arithmetic, the 16-bit ones with
+round-to-nearest stores and, as "f16-sr" / "bf16-sr", stochastic ones) and
+reports, per format, the relative L-inf error of the velocity field and the
+largest particle-centroid distance from the float64 run.
 """
 
 import math
@@ -31,8 +32,9 @@ def _run(cfg_json: dict, steps: int):
     return u, cx
 
 
-def compare(cfg_json: dict, steps: int, storages=("f32", "f16", "bf16")) -> Dict[str, dict]:
-    """Runs cfg_json with float64 storage and with each of `storages`."""
+def compare(cfg_json: dict, steps: int, storages=("f32", "f16", "bf16", "f16-sr", "bf16-sr")) -> Dict[str, dict]:
+    """Runs cfg_json with float64 storage and with each of `storages` (a
+    "-sr" suffix selects stochastic rounding of the 16-bit state)."""
     import copy
 
     ref = copy.deepcopy(cfg_json)
@@ -42,7 +44,9 @@ def compare(cfg_json: dict, steps: int, storages=("f32", "f16", "bf16")) -> Dict
     out = {"steps": steps, "max_abs_u_f64": scale}
     for st in storages:
         c = copy.deepcopy(cfg_json)
-        c["flow"]["storage"] = st
+        name, _, mode = st.partition("-")
+        c["flow"]["storage"] = name
+        c["flow"]["rounding"] = "stochastic" if mode == "sr" else "nearest"
         u, cx = _run(c, steps)
         err = float(np.max(np.abs(u - u64)) / scale) if scale > 0 else float(np.max(np.abs(u - u64)))
         cxe = max((float(np.max(np.abs(a - b))) for a, b in zip(cx, cx64)), default=0.0)

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define FUNLBM_ABI_VERSION 1
+#define FUNLBM_ABI_VERSION 2
 
 /* Boundary codes: the values of the reference enum BoundaryCondition
  * (src/funlbm/config/base.py:22-28). */
@@ -53,6 +53,14 @@ extern "C" {
 #define FUNLBM_STORE_BF16 2 /* (f - w_k*rho0)*dev_scale as bfloat16 */
 #define FUNLBM_STORE_F64 3  /* f as double, arithmetic in double */
 
+/* Rounding of the F16/BF16 state at each fused step's store. */
+#define FUNLBM_ROUND_NEAREST 0    /* round to nearest even */
+#define FUNLBM_ROUND_STOCHASTIC 1 /* round up with probability equal to the discarded
+                                     fraction of a storage ulp (gfx950's stochastic
+                                     conversion), fed by a hash of (seed, node, direction):
+                                     unbiased, so increments below half an ulp of the
+                                     stored deviation accumulate instead of being lost */
+
 typedef struct funlbm_lattice {
   int32_t q;          /* 13, 15, 19 or 27 */
   int32_t nx, ny, nz; /* node counts */
@@ -65,6 +73,11 @@ typedef struct funlbm_lattice {
   float dev_scale;    /* scale of deviation storage (ignored for F32/F64) */
   float body_force[3];/* uniform force added to FOL everywhere (extension; 0 = none) */
   float uw[6][3];     /* wall velocity of WALL_WITH_SPEED faces */
+  int32_t rounding;   /* FUNLBM_ROUND_*; STOCHASTIC needs F16 or BF16 storage */
+  uint32_t seed;      /* stochastic rounding: the dither seed of this call's step
+                         (a solve passes base + steps_done * 0x9E3779B9, mod 2^32, so
+                         every step draws new dithers); funlbm_case_pack takes the
+                         base and the batched step adds its own step count */
 } funlbm_lattice;
 
 /* ABI version (FUNLBM_ABI_VERSION).  No reference counterpart. */
@@ -73,11 +86,12 @@ int funlbm_abi_version(void);
 /* Thread-local description of the last error.  No reference counterpart. */
 const char* funlbm_last_error(void);
 
-/* Bytes of one fused-state buffer for `lat`: q planes of P elements of
- * lat->storage.  P = nodes rounded up to 64 plus 64 on grids under 65536
- * nodes, else nodes rounded up to 256 plus 4096 (so that the planes do not
- * start at power-of-two distances); P > nodes always.  Plane k of the state
- * starts at element k * P.  No reference counterpart. */
+/* Bytes of one fused-state buffer for `lat`.  The nodes are grouped into
+ * tiles of T consecutive nodes (T = 4096 from 2^20 nodes up, else 256); a tile
+ * holds its q direction planes of T elements one after another, and one slack
+ * tile precedes the first tile and one follows the last:
+ * (ceil(n / T) + 2) * T * q elements of lat->storage.  Element (k, node j)
+ * sits at q T + ((j / T) q + k) T + j % T.  No reference counterpart. */
 int funlbm_state_bytes(const funlbm_lattice* lat, int64_t* bytes);
 
 /* ---- reference-structured operations (explicit path, float, in place) ---- */

@@ -56,3 +56,114 @@ def deviation_round(f, w, storage, scale=1024.0, rho0=1.0):
     else:
         raise ValueError(storage)
     return (r * inv + wr).astype(np.float32)
+
+
+# ---------------------------------------------------------------- 16-bit state
+# Restatements, for the tests, of the fused state's tile layout and of the
+# stochastic rounding's dither and conversion (lbm_common.h).
+_M32 = 0xFFFFFFFF
+
+
+def mix32(x):
+    """The 32-bit hash of the dither (x ^= x >> 16; x *= 0x7feb352d;
+    x ^= x >> 15; x *= 0x846ca68b; x ^= x >> 16) on uint64 arrays."""
+    x = np.asarray(x, dtype=np.uint64) & np.uint64(_M32)
+    x = x ^ (x >> np.uint64(16))
+    x = (x * np.uint64(0x7FEB352D)) & np.uint64(_M32)
+    x = x ^ (x >> np.uint64(15))
+    x = (x * np.uint64(0x846CA68B)) & np.uint64(_M32)
+    x = x ^ (x >> np.uint64(16))
+    return x
+
+
+DITHER_MULT = (0x8990C1, 0x932181, 0xF1E9DD, 0xFE5ED1, 0xC5D53F, 0xE3D3BB, 0x8BA1E7, 0xD35FBB, 0x9FD5AB,
+               0x8BAA7D, 0xA8F9FB, 0xDC746D, 0xBFCFF1, 0xF63447, 0xBD4A6F, 0xA56AE5, 0xABDD59, 0xBB2F91,
+               0x8001ED, 0xDD7671, 0xBE2E8F, 0xDBCAC3, 0x9006FB, 0xCE98A9, 0xFF9FE3, 0xBC09CB, 0xDC5B09)
+
+
+def dither_mult(k):
+    return DITHER_MULT[k]
+
+
+def step_seed(base, steps_done):
+    return (int(base) + int(steps_done) * 0x9E3779B9) & _M32
+
+
+def dither_words(n, q, seed):
+    """uint64 [q, n]: the 32-bit dither word of direction k at node j in a step
+    of seed `seed`: ((h mod 2^24) * C_k) mod 2^32, h = mix32(j ^ mix32(seed))."""
+    h = mix32(np.arange(n, dtype=np.uint64) ^ mix32(np.uint64(seed)))
+    h24 = h & np.uint64(0xFFFFFF)
+    return np.stack([(h24 * np.uint64(dither_mult(k))) & np.uint64(_M32) for k in range(q)])
+
+
+def sr_round16(d, r, storage):
+    """Stochastic rounding of float32 d to f16 / bf16 bit patterns (uint16):
+    truncate toward zero at the format's precision, then add one ulp iff the
+    discarded fraction, truncated to 13 (f16) or 16 (bf16) bits, plus the top
+    13 / 16 bits of the 32-bit word r reach one ulp."""
+    d = np.asarray(d, dtype=np.float32)
+    x = np.abs(d.astype(np.float64))
+    bits = 13 if storage == "f16" else 16
+    mant = 10 if storage == "f16" else 7
+    _, e2 = np.frexp(x)
+    e = e2.astype(np.int64) - 1  # x in [2^e, 2^(e+1))
+    if storage == "f16":
+        e = np.maximum(e, -14)  # subnormal f16: ulp 2^-24
+    ulp_e = e - mant
+    qv = np.ldexp(x, -ulp_e)
+    fl = np.floor(qv)
+    rem = np.floor((qv - fl) * float(1 << bits))
+    rh = (np.asarray(r, dtype=np.uint64) >> np.uint64(32 - bits)).astype(np.float64)
+    up = (rem + rh) >= float(1 << bits)
+    mag = np.ldexp(fl + up, ulp_e)
+    val = np.copysign(mag, d.astype(np.float64))
+    with np.errstate(over="ignore"):  # beyond the f16 range: infinity, as the hardware
+        if storage == "f16":
+            return val.astype(np.float16).view(np.uint16)
+        return (val.astype(np.float32).view(np.uint32) >> np.uint32(16)).astype(np.uint16)
+
+
+def rne_round16(d, storage):
+    """Round-to-nearest-even of float32 d to f16 / bf16 bit patterns."""
+    d = np.asarray(d, dtype=np.float32)
+    if storage == "f16":
+        return d.astype(np.float16).view(np.uint16)
+    u = d.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def state_tile_shift(n):
+    return 12 if n >= (1 << 20) else 8
+
+
+def state_elems(q, n):
+    T = 1 << state_tile_shift(n)
+    return (-(-n // T) + 2) * T * q
+
+
+def state_index(q, n):
+    """int64 [q, n]: element of (direction k, node j) in a fused-state buffer."""
+    ts = state_tile_shift(n)
+    T = 1 << ts
+    j = np.arange(n, dtype=np.int64)
+    k = np.arange(q, dtype=np.int64)[:, None]
+    return (q + (j >> ts) * q + k) * T + (j & (T - 1))
+
+
+def deviation_round_sr(f, w, storage, seed, scale=1024.0):
+    """The state storage's round trip with stochastic rounding of the step of
+    seed `seed`: f [X, Y, Z, Q] float32, nodes in the kernels' order."""
+    f = np.asarray(f, dtype=np.float32)
+    q = f.shape[-1]
+    n = int(np.prod(f.shape[:-1]))
+    wr = np.asarray(w, dtype=np.float32)
+    s = np.float32(scale)
+    d = ((f - wr) * s).astype(np.float32).reshape(n, q).T
+    bits = sr_round16(d, dither_words(n, q, seed), storage)
+    if storage == "f16":
+        r = bits.view(np.float16).astype(np.float32)
+    else:
+        r = (bits.astype(np.uint32) << np.uint32(16)).view(np.float32)
+    return (r.T.reshape(f.shape) * (np.float32(1) / s) + wr).astype(np.float32)

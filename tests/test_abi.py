@@ -36,9 +36,38 @@ def test_bindings_cover_header():
 
 def test_abi_version_and_struct_size():
     lib = nat.lib()
-    assert lib.funlbm_abi_version() == 1
-    # 5 ints + 6 ints + 1 int + 3 floats + 3 floats + 18 floats
-    assert ctypes.sizeof(nat.Lattice) == 4 * (5 + 6 + 1 + 3 + 3 + 18)
+    assert lib.funlbm_abi_version() == 2
+    # 5 ints + 6 ints + 1 int + 3 floats + 3 floats + 18 floats + rounding + seed
+    assert ctypes.sizeof(nat.Lattice) == 4 * (5 + 6 + 1 + 3 + 3 + 18 + 2)
+
+
+def test_rounding_modes_are_checked_on_the_host():
+    lat = nat.Lattice()
+    lat.q = 19
+    lat.nx = lat.ny = lat.nz = 8
+    lat.tau = 0.7
+    lat.rho0 = 1.0
+    lat.dev_scale = 1024.0
+    for i in range(6):
+        lat.bc[i] = nat.BC_WALL
+    nbytes = ctypes.c_int64()
+    lat.storage = nat.STORE_F32
+    lat.rounding = nat.ROUND_STOCHASTIC
+    assert nat.lib().funlbm_state_bytes(ctypes.byref(lat), ctypes.byref(nbytes)) == nat.EINVAL
+    assert "F16/BF16" in nat.last_error()
+    lat.rounding = 7
+    lat.storage = nat.STORE_F16
+    assert nat.lib().funlbm_state_bytes(ctypes.byref(lat), ctypes.byref(nbytes)) == nat.EINVAL
+    for storage in (nat.STORE_F16, nat.STORE_BF16):
+        lat.storage = storage
+        lat.rounding = nat.ROUND_STOCHASTIC
+        assert nat.lib().funlbm_state_bytes(ctypes.byref(lat), ctypes.byref(nbytes)) == 0
+
+
+def test_step_seed_matches_the_kernels_rule():
+    assert nat.step_seed(0, 0) == 0
+    assert nat.step_seed(5, 1) == 5 + 0x9E3779B9
+    assert nat.step_seed(0xFFFFFFFF, 2) == (0xFFFFFFFF + 2 * 0x9E3779B9) % (1 << 32)
 
 
 def test_argument_errors_need_no_device():

@@ -20,7 +20,7 @@ def sphere_case(nodes, q=19, tau=0.6, r=4.0, rou=1.5, steps=30, center=None):
                     gravity=(0.0, 0.0, 2e-5))
 
 
-def lbmd3_for(spec, storage):
+def lbmd3_for(spec, storage, rounding="nearest"):
     from funlbm_amd.lbm import LBMD3, Config
 
     faces = {f: {"code": "WALL"} for f in ("input", "output", "back", "forward", "bottom", "top")}
@@ -28,7 +28,7 @@ def lbmd3_for(spec, storage):
     cfg = Config().from_json({
         "flow": {"param_type": f"D3Q{spec.q}", "size": [n - 1 for n in spec.nodes],
                  "param": {"mu": (float(np.float32(spec.tau)) - 0.5) / 3, "gravity": list(spec.gravity)},
-                 "boundary": faces, "storage": storage},
+                 "boundary": faces, "storage": storage, "rounding": rounding, "seed": spec.seed},
         "particles": [{"type": "sphere", "r": p.r, "rou": p.rou,
                        "coord": {"center": list(p.center)}}],
         "log_every": 0,
@@ -38,12 +38,16 @@ def lbmd3_for(spec, storage):
     return lbm
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16"])
-def test_batch_case_equals_single_lattice(storage):
+@pytest.mark.parametrize("storage,rounding", [("f32", "nearest"), ("f16", "nearest"), ("f16", "stochastic")])
+def test_batch_case_equals_single_lattice(storage, rounding):
+    """With stochastic rounding the case's dither follows its own step count
+    from its seed, as the single-lattice solve's does."""
     spec = sphere_case((20, 22, 26), steps=25)
-    lbm = lbmd3_for(spec, storage)
+    spec.seed = 99
+    lbm = lbmd3_for(spec, storage, rounding)
     spec.tau = lbm.flow.lat.tau  # the solver's tau comes from mu; use the same float32
-    eng = BatchEngine(storage=storage, max_nodes=1 << 16, max_markers=4096, max_particles=8, max_cases=4)
+    eng = BatchEngine(storage=storage, max_nodes=1 << 16, max_markers=4096, max_particles=8, max_cases=4,
+                      rounding=rounding)
     cid = eng.add(spec)
     eng.step(25)
     assert lbm.flow.lat.tau == eng.slots[cid].lat.tau
@@ -69,13 +73,18 @@ def _mixed_specs():
     ]
 
 
-def test_alone_equals_mixed_bitwise():
+@pytest.mark.parametrize("rounding", ["nearest", "stochastic"])
+def test_alone_equals_mixed_bitwise(rounding):
     specs = _mixed_specs()
-    mix = BatchEngine(storage="f16", max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8)
+    for i, s in enumerate(specs):
+        s.seed = 1000 + i
+    mix = BatchEngine(storage="f16", max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8,
+                      rounding=rounding)
     ids = [mix.add(s) for s in specs]
     mix.step(40)
     for s, cid in zip(specs, ids):
-        alone = BatchEngine(storage="f16", max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8)
+        alone = BatchEngine(storage="f16", max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8,
+                            rounding=rounding)
         a = alone.add(s)
         alone.step(40)
         ua, ra = alone.fields(a)

@@ -11,7 +11,7 @@ import pytest
 import torch
 
 from oracle import lbm_oracle as orc
-from tests.helpers import deviation_round, flow_json, random_state, rel_linf
+from tests.helpers import deviation_round, deviation_round_sr, flow_json, random_state, rel_linf, step_seed
 
 pytestmark = pytest.mark.gpu
 
@@ -207,17 +207,24 @@ def test_fused_bitwise_equal_to_oracle(q, size):
     assert np.array_equal(got.view(np.uint32), st.f.view(np.uint32)), rel_linf(got, st.f)
 
 
-def run_oracle_rounded(q, size, mu, bcs, steps, f0, storage, body_force=None, scale=1024.0):
+def run_oracle_rounded(q, size, mu, bcs, steps, f0, storage, body_force=None, scale=1024.0,
+                       rounding="nearest", seed=0):
     """The oracle's steps with the state storage's rounding applied to the
-    post-collision populations, where the fused step stores them."""
+    post-collision populations, where the fused path stores them: the first
+    collision's store (the packing of the explicit state) rounds to nearest,
+    the store of step i + 1's collision by the solver's i-th fused step
+    (i >= 1) rounds as `rounding` with the dither of step_seed(seed, i - 1)."""
     nodes = tuple(int(s) + 1 for s in size)
     st = orc.init_flow(q, nodes, mu=mu, bc=bcs, body_force=body_force)
     orc.init_solver(st)
     st.f = f0.copy()
     orc.update_u_rou(st)
-    for _ in range(steps):
+    for i in range(steps):
         orc.cul_equ(st)
-        st.f = deviation_round(st.f, st.lat.w, storage, scale)
+        if rounding == "stochastic" and i > 0:
+            st.f = deviation_round_sr(st.f, st.lat.w, storage, step_seed(seed, i - 1), scale)
+        else:
+            st.f = deviation_round(st.f, st.lat.w, storage, scale)
         orc.f_stream(st)
         orc.update_u_rou(st)
         st.FOL = np.zeros_like(st.FOL)
@@ -226,23 +233,25 @@ def run_oracle_rounded(q, size, mu, bcs, steps, f0, storage, body_force=None, sc
     return st
 
 
+@pytest.mark.parametrize("rounding", ["nearest", "stochastic"])
 @pytest.mark.parametrize("size", [(9, 10, 15), (11, 9, 13)])
 @pytest.mark.parametrize("storage", ["f16", "bf16"])
 @pytest.mark.parametrize("q", [19, 27])
-def test_deviation_storage_matches_rounded_oracle(q, storage, size):
+def test_deviation_storage_matches_rounded_oracle(q, storage, size, rounding):
     """f16/bf16 state: the fused path against the oracle with the storage's
-    round trip applied after every collision.  The 16-bit paths compute on
-    the scaled deviations with reciprocals and FMAs (float32 rounding, far
-    below the storage's 2^-11 or 2^-8), so a rounding of the state can land on
-    the other side now and then: after 30 steps f stays within two storage ulps
-    of the largest deviation |f - w| of the run, the initial one (relative to
+    round trip applied after every collision (to nearest, or stochastically
+    with the solver's dither words).  The 16-bit paths compute on the scaled
+    deviations with reciprocals and FMAs (float32 rounding, far below the
+    storage's 2^-11 or 2^-8), so a rounding of the state can land on the other
+    side now and then: after 30 steps f stays within two storage ulps of the
+    largest deviation |f - w| of the run, the initial one (relative to
     max|f|)."""
     bcs = BC_SETS["channel"]
     nodes = tuple(s + 1 for s in size)
     f0 = random_state(q, nodes, seed=6)
     g = [2e-5, 0.0, -1e-5]
-    lbm = run_solver(q, size, 0.07, bcs, 30, f0=f0, storage=storage, body_force=g)
-    st = run_oracle_rounded(q, size, 0.07, bcs, 30, f0, storage, body_force=g)
+    lbm = run_solver(q, size, 0.07, bcs, 30, f0=f0, storage=storage, body_force=g, rounding=rounding, seed=77)
+    st = run_oracle_rounded(q, size, 0.07, bcs, 30, f0, storage, body_force=g, rounding=rounding, seed=77)
     got = lbm.flow.f.cpu().numpy()
     err = rel_linf(got, st.f)
     bits = {"f16": 11, "bf16": 8}[storage]

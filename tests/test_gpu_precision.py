@@ -20,6 +20,8 @@ def test_lid_driven_cavity_storage_errors():
     assert res["f32"]["u_rel_linf"] < 5e-5
     assert res["f16"]["u_rel_linf"] < 1e-3
     assert res["bf16"]["u_rel_linf"] < 2e-2
+    assert res["f16-sr"]["u_rel_linf"] < 1e-3
+    assert res["bf16-sr"]["u_rel_linf"] < 2e-2
 
 
 def test_poiseuille_storage_errors():
@@ -29,6 +31,11 @@ def test_poiseuille_storage_errors():
     assert res["f32"]["u_rel_linf"] < 1e-4
     assert res["f16"]["u_rel_linf"] < 5e-3
     assert res["bf16"]["u_rel_linf"] < 0.25
+    # stochastic stores keep the sub-ulp forcing increments that nearest
+    # rounding drops
+    assert res["f16-sr"]["u_rel_linf"] < 1e-3
+    assert res["f16-sr"]["u_rel_linf"] < 0.5 * res["f16"]["u_rel_linf"]
+    assert res["bf16-sr"]["u_rel_linf"] < 0.5 * res["bf16"]["u_rel_linf"]
 
 
 def test_settling_sphere_centroid_errors():
