@@ -142,7 +142,8 @@ def storage_leg(size, q, storage, steps, warmup, ref_mlups=None, rounding=None):
     elem = {"f32": 4, "f16": 2, "bf16": 2, "f64": 8}[storage]
     mlups = n / (ms * 1e-3) / 1e6
     ach = n * 2 * q * elem / (kms * 1e-3) / 1e9
-    rd = f", {rounding} rounding" if rounding else ""
+    eff = rounding or ("stochastic" if storage in ("f16", "bf16") else None)
+    rd = f", {eff} rounding" if eff else ""
     out = {"workload": f"D3Q{q} {size}^3 periodic, tau 0.6, {storage} state{rd}", "mlups": round(mlups, 1),
            "ms_per_step": round(ms, 4), "kernel_ms": round(kms, 4), "bytes_per_lu": 2 * q * elem,
            "achieved_gbs": round(ach, 1), "frac_of_8tbs": round(ach / HBM_PEAK_GBS, 4)}
