@@ -58,12 +58,28 @@ __device__ __forceinline__ void after_pull(const KParams& p, int64_t node, const
 // The post-step f, u, rho of a node (the reference's tensors after the step),
 // without the collision.  Shared by fused_node and the observers below.
 template <class L, class S, bool FORCE>
+__device__ __forceinline__ void step_node_dev(const KParams& p, int64_t node, const Node& c, const FusedIO<S>& io,
+                                              bool new_here, bool old_here, float* g, float& rho, float* u,
+                                              float* drho = nullptr);
+
+// A 16-bit state's nodes take the deviation arithmetic of its fused step, so
+// that u and rho come out of sums of the small deviations, not of the
+// populations w_k + g/scale (whose float32 rounding would cost ~1e-8 of u).
+template <class L, class S, bool FORCE>
 __device__ __forceinline__ void step_node(const KParams& p, int64_t node, const Node& c,
                                           const FusedIO<S>& io, bool new_here, bool old_here,
                                           typename S::T* f, typename S::T& rho,
                                           typename S::T* u) {
-  pull_node<L, S>(p, io.src, c, node, io.rho_prev, f);
-  after_pull<L, S, FORCE>(p, node, c, io, new_here, old_here, f, rho, u);
+  if constexpr (S::fast) {
+    step_node_dev<L, S, FORCE>(p, node, c, io, new_here, old_here, f, rho, u);
+    static_for<0, L::Q>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      f[K] = __builtin_fmaf(f[K], p.inv_scale, wt<float>(L::W[K]));
+    });
+  } else {
+    pull_node<L, S>(p, io.src, c, node, io.rho_prev, f);
+    after_pull<L, S, FORCE>(p, node, c, io, new_here, old_here, f, rho, u);
+  }
 }
 
 // Collision of a stepped node and the store of its post-collision
@@ -125,16 +141,14 @@ __device__ __forceinline__ void fused_node(const KParams& p, int64_t node, const
   }
 }
 
-// The fused step of a 16-bit state in deviation arithmetic (moments_dev,
-// add_forcing_dev, collide_dev): the populations stay scaled deviations from
-// the load to the store, which rounds to nearest or, with p.sr, stochastically
-// with the dither of this step's seed mix (mix32 of the step's seed).
+// Pull, forcing and moments of a node of a 16-bit state in deviation
+// arithmetic (moments_dev, add_forcing_dev): g holds the scaled deviations
+// (f - w_k) scale of the pulled populations after the forcing.
 template <class L, class S, bool FORCE>
-__device__ __forceinline__ void fused_node_dev(const KParams& p, int64_t node, const FusedIO<S>& io,
-                                               bool new_here, bool old_here, uint32_t seed_mix) {
+__device__ __forceinline__ void step_node_dev(const KParams& p, int64_t node, const Node& c, const FusedIO<S>& io,
+                                              bool new_here, bool old_here, float* g, float& rho, float* u,
+                                              float* drho) {
   using D = DevOf<S>;
-  const Node c = node_coords(p, node);
-  float g[L::Q];
   pull_node<L, D>(p, io.src, c, node, io.rho_prev, g);
   float Fn[3] = {0.0f, 0.0f, 0.0f};
   if (FORCE) {
@@ -149,9 +163,21 @@ __device__ __forceinline__ void fused_node_dev(const KParams& p, int64_t node, c
       add_forcing_dev<L>(p, g, u1, Fn);
     }
   }
-  float rho, u[3];
-  moments_dev<L>(p, c, g, Fn, rho, u);
-  collide_dev<L>(p, g, u, rho);
+  moments_dev<L>(p, c, g, Fn, rho, u, drho);
+}
+
+// The fused step of a 16-bit state in deviation arithmetic (step_node_dev,
+// collide_dev): the populations stay scaled deviations from the load to the
+// store, which rounds to nearest or, with p.sr, stochastically with the
+// dither of this step's seed mix (mix32 of the step's seed).
+template <class L, class S, bool FORCE>
+__device__ __forceinline__ void fused_node_dev(const KParams& p, int64_t node, const FusedIO<S>& io,
+                                               bool new_here, bool old_here, uint32_t seed_mix) {
+  const Node c = node_coords(p, node);
+  float g[L::Q];
+  float rho, u[3], drho;
+  step_node_dev<L, S, FORCE>(p, node, c, io, new_here, old_here, g, rho, u, &drho);
+  collide_dev<L>(p, g, u, rho, drho);
   if (p.has_moving && io.rho_next) {
     bool mf = false;
 #pragma unroll

@@ -117,7 +117,16 @@ __device__ inline void interp_marker(const KParams& p, int m, int lane,
   if (lw.valid) {
     const int64_t node = lin(p, lw.x, lw.y, lw.z);
     float uu[3], rr;
-    if constexpr (FROM_STATE) {
+    if constexpr (FROM_STATE && S::fast) {
+      // a 16-bit state: moments of the scaled deviations, as its fused step
+      const Node c{lw.x, lw.y, lw.z};
+      float g[L::Q];
+      pull_node<L, DevOf<S>, false>(p, state, c, node, rho_face, g);
+      float F[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) F[d] = fol_old ? fol_old[(int64_t)d * p.n + node] + p.g[d] : p.g[d];
+      moments_dev<L>(p, c, g, F, rr, uu);
+    } else if constexpr (FROM_STATE) {
       const Node c{lw.x, lw.y, lw.z};
       T f[L::Q];
       pull_node<L, S, false>(p, state, c, node, rho_face, f);

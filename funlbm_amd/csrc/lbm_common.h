@@ -782,9 +782,12 @@ __device__ inline void add_forcing(T* f, const T u[3], const T F[3], const Relax
 // same operations as the FAST path on f, without converting each population
 // to f and back (38 fewer instructions per node), and on smaller numbers.
 // ---------------------------------------------------------------------------
+// drho (optional): rho - 1 from the sum of the deviations, to float32's
+// relative precision; rho, held as 1 + drho in float32, resolves drho only to
+// about 6e-8.
 template <class L>
 __device__ inline void moments_dev(const KParams& p, const Node& c, const float* g, const float F[3], float& rho,
-                                   float u[3]) {
+                                   float u[3], float* drho = nullptr) {
   // each sum starts from its first term (no leading zero to add)
   constexpr int fx = first_along<L>(0), fy = first_along<L>(1), fz = first_along<L>(2);
   float r = g[0];
@@ -802,6 +805,7 @@ __device__ inline void moments_dev(const KParams& p, const Node& c, const float*
     if constexpr (k != fz && L::E[k][2] == -1) jz = jz - g[k];
   });
   rho = __builtin_fmaf(r, p.inv_scale, 1.0f);
+  if (drho) *drho = r * p.inv_scale;
   const float rr = __builtin_amdgcn_rcpf(rho);
   u[0] = __builtin_fmaf(F[0], 0.5f, jx * p.inv_scale) * rr;
   u[1] = __builtin_fmaf(F[1], 0.5f, jy * p.inv_scale) * rr;
@@ -815,14 +819,17 @@ __device__ inline void moments_dev(const KParams& p, const Node& c, const float*
 
 // g <- g (1 - 1/tau) + (s/tau) w_k (rho a_k - 1), the BGK relaxation towards
 // g_eq = s w_k (rho a_k - 1), a_k = 1 + t + t^2/2 - 3|u|^2/2, t = 3 e_k.u.
-// With c0 = rho (1 - 3|u|^2/2) - 1, rho a_k - 1 = c0 + rho t (1 + t/2).  A
-// direction and its reverse (t -> -t) share, with sw = (s/tau) w_k,
-// anti = sw rho t = (3 sw rho)(e_k.u) and sym = sw c0 + anti (3/2)(e_k.u), and
-// take sym +- anti; 3 sw rho and sw c0 are formed once per weight.
+// With c0 = rho (1 - 3|u|^2/2) - 1 = drho - (3/2)|u|^2 rho, rho a_k - 1 =
+// c0 + rho t (1 + t/2).  c0 is formed from drho = rho - 1 (the sum of the
+// deviations), not from rho: at slow flow its two terms lie far below
+// float32's resolution of rho near 1.  A direction and its reverse
+// (t -> -t) share, with sw = (s/tau) w_k, anti = sw rho t = (3 sw rho)(e_k.u)
+// and sym = sw c0 + anti (3/2)(e_k.u), and take sym +- anti; 3 sw rho and
+// sw c0 are formed once per weight.
 template <class L>
-__device__ inline void collide_dev(const KParams& p, float* g, const float u[3], float rho) {
+__device__ inline void collide_dev(const KParams& p, float* g, const float u[3], float rho, float drho) {
   const float uu = __builtin_fmaf(u[2], u[2], __builtin_fmaf(u[1], u[1], u[0] * u[0]));
-  const float c0 = __builtin_fmaf(rho, __builtin_fmaf(uu, -1.5f, 1.0f), -1.0f);
+  const float c0 = __builtin_fmaf(uu * -1.5f, rho, drho);
   const float sr = p.scale * p.rtau;  // s / tau
   const float keep = 1.0f - p.rtau;
   static_for<0, L::Q>([&](auto kc) {
