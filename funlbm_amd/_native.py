@@ -91,6 +91,7 @@ SIGNATURES = {
     "funlbm_equilibrium_state": (c_int32, [LATP, P, P, P, P]),
     "funlbm_init_state": (c_int32, [LATP, P, P]),
     "funlbm_fused_step": (c_int32, [LATP, P, P, P, P, P, P, P, P, P]),
+    "funlbm_fused_step_stats": (c_int32, [LATP, P, P, P, P, P, P, P, P, P, c_int32, P, P]),
     "funlbm_materialize": (c_int32, [LATP, P, P, P, P, P, P, P, P, P, P, P]),
     "funlbm_copy_calibrate": (c_int32, [P, P, c_int64, P]),
     "funlbm_ibm_interp": (

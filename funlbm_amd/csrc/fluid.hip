@@ -275,6 +275,34 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) voi
                                         out_u, out_rho, out_p);
 }
 
+constexpr int kStatsStage = 256;  // first-stage blocks of the statistics' final reduction
+
+// The fused step that also reduces the reference's f, u, rho after this
+// step's stream and moments (what funlbm_flow_stats reads from the step's
+// source state) into one partial per block: the statistics of a logged step
+// without a second pass over the state.  Block b (after the XCD remap) takes
+// nodes [256 b, 256 b + 256) and writes partial b.
+template <class L, class S, bool FORCE>
+__global__ __launch_bounds__(kBlock) void k_fused_stats(FUNLBM_FUSED_ARGS, double* __restrict__ part) {
+  (void)p_arg;
+  const KParams& p = *(const KParams*)__builtin_amdgcn_kernarg_segment_ptr();
+  const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const int64_t node = (int64_t)b * kBlock + threadIdx.x;
+  NodeStats v;
+  stats_identity(v);
+  if (node < p.n) {
+    bool new_here = fol_new != nullptr, old_here = fol_old != nullptr;
+    if (FORCE && fol_tiles) {
+      const uint32_t entry = fol_tiles[(uint32_t)node >> 8], stamp = (uint32_t)ctl[0];
+      new_here = new_here && entry == stamp;
+      old_here = old_here && (entry == stamp || entry == stamp - 1u);
+    }
+    FusedIO<S> io{src, dst, fol_old, fol_new, rho_prev, rho_next, nullptr, nullptr, nullptr, nullptr};
+    fused_stats_node<L, S, FORCE>(p, node, io, new_here, old_here, p.seed_mix, v);
+  }
+  stats_block_reduce(v, part + (int64_t)b * kStats);
+}
+
 // Plain copy, one 16-byte element per thread (the fastest plain form measured
 // on the MI355X: about 6.1 TB/s read + write on 2 GiB).
 __global__ __launch_bounds__(kBlock) void k_copy(const uint4* __restrict__ src,
@@ -487,6 +515,52 @@ int funlbm_fused_step(const funlbm_lattice* lat, const void* state_src, void* st
                       const int32_t* ctl, const float* rho_prev, float* rho_next, void* stream) {
   return fused_launch(lat, state_src, state_dst, fol_old, fol_new, fol_tiles, ctl, rho_prev, rho_next,
                       nullptr, nullptr, nullptr, nullptr, false, stream);
+}
+
+int funlbm_fused_step_stats(const funlbm_lattice* lat, const void* state_src, void* state_dst,
+                            const float* fol_old, const float* fol_new, const uint32_t* fol_tiles,
+                            const int32_t* ctl, const float* rho_prev, float* rho_next, double* partials,
+                            int32_t n_partials, double* out, void* stream) {
+  KParams p;
+  int rc = make_params(lat, &p);
+  if (rc) return rc;
+  if ((rc = reject_neq(p, "fused_step_stats"))) return rc;
+  if (!state_src || !state_dst) return set_error(FUNLBM_EINVAL, "fused: NULL state");
+  if (state_src == state_dst) return set_error(FUNLBM_EINVAL, "fused: source and destination alias");
+  if (!partials || !out) return set_error(FUNLBM_EINVAL, "fused_step_stats: NULL statistics array");
+  if (p.has_moving && !rho_prev)
+    return set_error(FUNLBM_EINVAL, "fused: moving wall needs the previous face density");
+  if (fol_tiles && !ctl) return set_error(FUNLBM_EINVAL, "fused: a FOL tile table needs ctl");
+  const int blocks = grid_for(p.n);
+  const int stage = std::min(blocks, kStatsStage);
+  if ((int64_t)n_partials < (int64_t)blocks + stage)
+    return set_error(FUNLBM_EINVAL,
+                     "fused_step_stats: %d partials given, %d needed (one per 256 nodes, and %d more)", n_partials,
+                     blocks + stage, stage);
+  const bool force = fol_old || fol_new || p.has_body;
+  hipStream_t s = (hipStream_t)stream;
+  rc = dispatch_lattice(lat->q, [&](auto l) {
+    using L = decltype(l);
+    return dispatch_storage(lat->storage, [&](auto st) {
+      using S = decltype(st);
+      using E = typename S::elem;
+      auto go = [&](auto kern) {
+        hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, p, (const E*)state_src, (E*)state_dst, fol_old,
+                           fol_new, fol_tiles, ctl, rho_prev, rho_next, nullptr, nullptr, nullptr, nullptr,
+                           partials);
+        return check_launch("fused_step_stats");
+      };
+      return force ? go(k_fused_stats<L, S, true>) : go(k_fused_stats<L, S, false>);
+    });
+  });
+  if (rc) return rc;
+  // many partials: a first stage over `stage` blocks, then one block
+  double* second = partials + (int64_t)blocks * kStats;
+  hipLaunchKernelGGL(k_stats_stage, dim3(stage), dim3(kBlock), 0, s, partials, blocks, second);
+  if ((rc = check_launch("fused_step_stats"))) return rc;
+  hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(kBlock), 0, s, second, stage, (double)p.n * lat->q,
+                     (double)p.n * 3, (double)p.n, out);
+  return check_launch("fused_step_stats");
 }
 
 int funlbm_materialize(const funlbm_lattice* lat, const void* state_src, const float* fol_old,

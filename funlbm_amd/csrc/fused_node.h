@@ -166,17 +166,14 @@ __device__ __forceinline__ void step_node_dev(const KParams& p, int64_t node, co
   moments_dev<L>(p, c, g, Fn, rho, u, drho);
 }
 
-// The fused step of a 16-bit state in deviation arithmetic (step_node_dev,
-// collide_dev): the populations stay scaled deviations from the load to the
-// store, which rounds to nearest or, with p.sr, stochastically with the
-// dither of this step's seed mix (mix32 of the step's seed).
-template <class L, class S, bool FORCE>
-__device__ __forceinline__ void fused_node_dev(const KParams& p, int64_t node, const FusedIO<S>& io,
-                                               bool new_here, bool old_here, uint32_t seed_mix) {
-  const Node c = node_coords(p, node);
-  float g[L::Q];
-  float rho, u[3], drho;
-  step_node_dev<L, S, FORCE>(p, node, c, io, new_here, old_here, g, rho, u, &drho);
+// Collision of a 16-bit node in deviation arithmetic (collide_dev) and the
+// store of its populations, which rounds to nearest or, with p.sr,
+// stochastically with the dither of this step's seed mix (mix32 of the step's
+// seed).
+template <class L, class S>
+__device__ __forceinline__ void collide_store_dev(const KParams& p, int64_t node, const Node& c,
+                                                  const FusedIO<S>& io, float* g, float rho, float drho,
+                                                  const float* u, uint32_t seed_mix) {
   collide_dev<L>(p, g, u, rho, drho);
   if (p.has_moving && io.rho_next) {
     bool mf = false;
@@ -209,6 +206,247 @@ __device__ __forceinline__ void fused_node_dev(const KParams& p, int64_t node, c
       char* base = (char*)io.dst + ((int64_t)Q + (int64_t)tn * Q + K) * ((int64_t)1 << ts) * (int64_t)sizeof(R);
       __builtin_nontemporal_store(S::from_float(g[K]), (R*)(base + lb));
     });
+  }
+}
+
+// The fused step of a 16-bit state in deviation arithmetic (step_node_dev,
+// collide_store_dev): the populations stay scaled deviations from the load to
+// the store.
+template <class L, class S, bool FORCE>
+__device__ __forceinline__ void fused_node_dev(const KParams& p, int64_t node, const FusedIO<S>& io,
+                                               bool new_here, bool old_here, uint32_t seed_mix) {
+  const Node c = node_coords(p, node);
+  float g[L::Q];
+  float rho, u[3], drho;
+  step_node_dev<L, S, FORCE>(p, node, c, io, new_here, old_here, g, rho, u, &drho);
+  collide_store_dev<L, S>(p, node, c, io, g, rho, drho, u, seed_mix);
+}
+
+// Statistics of the reference's f, u and rho of one node (the values as
+// float32, as the reference's tensors hold them), accumulated per thread in
+// float for the extrema and in double for the sums.  The sum of a node's
+// populations is its density (the moments' sum), so the mean of f is taken
+// as the mean of rho over Q.
+constexpr int kStats = 9;  // partial: min f, max f, sum f, min u, max u, sum u, min rho, max rho, sum rho
+
+struct NodeStats {
+  float fmin, fmax, umin, umax, rmin, rmax;
+  double usum, rsum;
+};
+
+__device__ inline void stats_identity(NodeStats& a) {
+  a.fmin = a.umin = a.rmin = INFINITY;
+  a.fmax = a.umax = a.rmax = -INFINITY;
+  a.usum = a.rsum = 0.0;
+}
+
+template <int Q>
+__device__ inline void stats_add(NodeStats& a, const float* f, const float* u, float rho) {
+  float lo = f[0], hi = f[0];
+#pragma unroll
+  for (int k = 1; k < Q; ++k) {
+    lo = fminf(lo, f[k]);
+    hi = fmaxf(hi, f[k]);
+  }
+  a.fmin = fminf(a.fmin, lo);
+  a.fmax = fmaxf(a.fmax, hi);
+  a.umin = fminf(a.umin, fminf(fminf(u[0], u[1]), u[2]));
+  a.umax = fmaxf(a.umax, fmaxf(fmaxf(u[0], u[1]), u[2]));
+  a.usum += ((double)u[0] + (double)u[1]) + (double)u[2];
+  a.rmin = fminf(a.rmin, rho);
+  a.rmax = fmaxf(a.rmax, rho);
+  a.rsum += (double)rho;
+}
+
+// Wave-wide reduction, result in lane 63: inclusive scans within the rows of
+// 16 lanes (row_shr 1, 2, 4, 8), then the row totals across rows
+// (row_bcast 15 into rows 1 and 3, row_bcast 31 into rows 2 and 3); lanes
+// without a source take the identity.
+template <class Op>
+__device__ inline float wave_reduce_f(float v, float ident, Op op) {
+  const int id = __float_as_int(ident);
+  v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x111, 0xf, 0xf, false)));
+  v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x112, 0xf, 0xf, false)));
+  v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x114, 0xf, 0xf, false)));
+  v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x118, 0xf, 0xf, false)));
+  v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x142, 0xa, 0xf, false)));
+  v = op(v, __int_as_float(__builtin_amdgcn_update_dpp(id, __float_as_int(v), 0x143, 0xc, 0xf, false)));
+  return v;
+}
+
+template <int CTRL, int ROW_MASK>
+__device__ inline double dpp_shift_f64(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// Sum over the wave (identity 0.0, whose bits are all zero), result in lane 63.
+__device__ inline double wave_sum_d(double v) {
+  v += dpp_shift_f64<0x111, 0xf>(v);
+  v += dpp_shift_f64<0x112, 0xf>(v);
+  v += dpp_shift_f64<0x114, 0xf>(v);
+  v += dpp_shift_f64<0x118, 0xf>(v);
+  v += dpp_shift_f64<0x142, 0xa>(v);
+  v += dpp_shift_f64<0x143, 0xc>(v);
+  return v;
+}
+
+// Block reduction of the per-thread statistics into part[kStats] (call with
+// the whole block): DPP within each wave, then the four waves' totals in
+// fixed order.
+__device__ inline void stats_block_reduce(NodeStats a, double* __restrict__ part) {
+  auto mn = [](float x, float y) { return fminf(x, y); };
+  auto mx = [](float x, float y) { return fmaxf(x, y); };
+  a.fmin = wave_reduce_f(a.fmin, INFINITY, mn);
+  a.fmax = wave_reduce_f(a.fmax, -INFINITY, mx);
+  a.umin = wave_reduce_f(a.umin, INFINITY, mn);
+  a.umax = wave_reduce_f(a.umax, -INFINITY, mx);
+  a.rmin = wave_reduce_f(a.rmin, INFINITY, mn);
+  a.rmax = wave_reduce_f(a.rmax, -INFINITY, mx);
+  a.usum = wave_sum_d(a.usum);
+  a.rsum = wave_sum_d(a.rsum);
+  constexpr int W = kBlock / 64;
+  __shared__ NodeStats wv[W];
+  if ((threadIdx.x & 63) == 63) wv[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    NodeStats t = wv[0];
+#pragma unroll
+    for (int w = 1; w < W; ++w) {
+      t.fmin = fminf(t.fmin, wv[w].fmin);
+      t.fmax = fmaxf(t.fmax, wv[w].fmax);
+      t.umin = fminf(t.umin, wv[w].umin);
+      t.umax = fmaxf(t.umax, wv[w].umax);
+      t.rmin = fminf(t.rmin, wv[w].rmin);
+      t.rmax = fmaxf(t.rmax, wv[w].rmax);
+      t.usum += wv[w].usum;
+      t.rsum += wv[w].rsum;
+    }
+    part[0] = t.fmin;
+    part[1] = t.fmax;
+    part[2] = t.rsum;
+    part[3] = t.umin;
+    part[4] = t.umax;
+    part[5] = t.usum;
+    part[6] = t.rmin;
+    part[7] = t.rmax;
+    part[8] = t.rsum;
+  }
+}
+
+// Fixed-order tree over the block of double statistics (k_stats_final).
+__device__ inline void stats_block_reduce_d(double v[kStats], double* __restrict__ part) {
+  __shared__ double red[kStats][kBlock];
+#pragma unroll
+  for (int i = 0; i < kStats; ++i) red[i][threadIdx.x] = v[i];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+#pragma unroll
+      for (int i = 0; i < kStats; i += 3) {
+        red[i][threadIdx.x] = fmin(red[i][threadIdx.x], red[i][threadIdx.x + s]);
+        red[i + 1][threadIdx.x] = fmax(red[i + 1][threadIdx.x], red[i + 1][threadIdx.x + s]);
+        red[i + 2][threadIdx.x] += red[i + 2][threadIdx.x + s];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < kStats) part[threadIdx.x] = red[threadIdx.x][0];
+}
+
+// First stage of the reduction of many partials: block g takes partials
+// [g c, g c + c), c = ceil(nblocks / gridDim.x), thread-strided in fixed order
+// then the block tree, and writes partial g of `out`.
+static __global__ __launch_bounds__(kBlock) void k_stats_stage(const double* __restrict__ part, int nblocks,
+                                                               double* __restrict__ out) {
+  const int c = (nblocks + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int b0 = (int)blockIdx.x * c;
+  const int b1 = min(nblocks, b0 + c);
+  double v[kStats];
+#pragma unroll
+  for (int i = 0; i < kStats; i += 3) {
+    v[i] = INFINITY;
+    v[i + 1] = -INFINITY;
+    v[i + 2] = 0.0;
+  }
+  for (int b = b0 + (int)threadIdx.x; b < b1; b += kBlock) {
+#pragma unroll
+    for (int i = 0; i < kStats; i += 3) {
+      v[i] = fmin(v[i], part[(int64_t)b * kStats + i]);
+      v[i + 1] = fmax(v[i + 1], part[(int64_t)b * kStats + i + 1]);
+      v[i + 2] += part[(int64_t)b * kStats + i + 2];
+    }
+  }
+  stats_block_reduce_d(v, out + (int64_t)blockIdx.x * kStats);
+}
+
+// Fixed-order reduction of nblocks partials (one block of kBlock threads:
+// thread t takes partials t, t + kBlock, ..., then the block tree) into
+// out = f min, mean, max, u min, mean, max, rho min, mean, max.
+static __global__ __launch_bounds__(kBlock) void k_stats_final(const double* __restrict__ part, int nblocks,
+                                                               double count_f, double count_u, double count_r,
+                                                               double* __restrict__ out) {
+  double v[kStats];
+#pragma unroll
+  for (int i = 0; i < kStats; i += 3) {
+    v[i] = INFINITY;
+    v[i + 1] = -INFINITY;
+    v[i + 2] = 0.0;
+  }
+  for (int b = threadIdx.x; b < nblocks; b += kBlock) {
+#pragma unroll
+    for (int i = 0; i < kStats; i += 3) {
+      v[i] = fmin(v[i], part[(int64_t)b * kStats + i]);
+      v[i + 1] = fmax(v[i + 1], part[(int64_t)b * kStats + i + 1]);
+      v[i + 2] += part[(int64_t)b * kStats + i + 2];
+    }
+  }
+  __shared__ double tot[kStats];
+  stats_block_reduce_d(v, tot);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = tot[0];
+    out[1] = tot[2] / count_f;
+    out[2] = tot[1];
+    out[3] = tot[3];
+    out[4] = tot[5] / count_u;
+    out[5] = tot[4];
+    out[6] = tot[6];
+    out[7] = tot[8] / count_r;
+    out[8] = tot[7];
+  }
+}
+
+// The fused step of one node that also adds the reference's f, u, rho after
+// this step's stream and moments (the fields the step collides) to v.
+template <class L, class S, bool FORCE>
+__device__ __forceinline__ void fused_stats_node(const KParams& p, int64_t node, const FusedIO<S>& io,
+                                                 bool new_here, bool old_here, uint32_t seed_mix,
+                                                 NodeStats& v) {
+  using T = typename S::T;
+  const Node c = node_coords(p, node);
+  if constexpr (S::fast) {
+    float g[L::Q], f[L::Q];
+    float rho, u[3], drho;
+    step_node_dev<L, S, FORCE>(p, node, c, io, new_here, old_here, g, rho, u, &drho);
+    static_for<0, L::Q>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      f[K] = __builtin_fmaf(g[K], p.inv_scale, wt<float>(L::W[K]));
+    });
+    stats_add<L::Q>(v, f, u, rho);
+    collide_store_dev<L, S>(p, node, c, io, g, rho, drho, u, seed_mix);
+  } else {
+    T f[L::Q], rho, u[3];
+    step_node<L, S, FORCE>(p, node, c, io, new_here, old_here, f, rho, u);
+    float ff[L::Q], uf[3];
+#pragma unroll
+    for (int k = 0; k < L::Q; ++k) ff[k] = (float)f[k];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) uf[d] = (float)u[d];
+    stats_add<L::Q>(v, ff, uf, (float)rho);
+    collide_store<L, S>(p, node, c, io, f, rho, u);
   }
 }
 

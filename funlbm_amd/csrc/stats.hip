@@ -10,9 +10,8 @@
 
 namespace funlbm {
 
-// partial layout per block: min f, max f, sum f, min u, max u, sum u,
-// min rho, max rho, sum rho (double)
-constexpr int kStats = 9;
+// partial layout per block (kStats, fused_node.h): min f, max f, sum f,
+// min u, max u, sum u, min rho, max rho, sum rho (double)
 
 template <class L, class S, bool FORCE>
 __global__ __launch_bounds__(kBlock) void k_stats(
@@ -20,7 +19,8 @@ __global__ __launch_bounds__(kBlock) void k_stats(
     const float* __restrict__ fol_new, const uint32_t* __restrict__ fol_tiles,
     const int* __restrict__ ctl, const float* __restrict__ rho_prev, double* __restrict__ part) {
   using T = typename S::T;
-  double v[kStats] = {INFINITY, -INFINITY, 0.0, INFINITY, -INFINITY, 0.0, INFINITY, -INFINITY, 0.0};
+  NodeStats v;
+  stats_identity(v);
   for (int64_t node = (int64_t)blockIdx.x * kBlock + threadIdx.x; node < p.n;
        node += (int64_t)gridDim.x * kBlock) {
     bool new_here = fol_new != nullptr, old_here = fol_old != nullptr;
@@ -34,64 +34,14 @@ __global__ __launch_bounds__(kBlock) void k_stats(
     T f[L::Q], rho, u[3];
     step_node<L, S, FORCE>(p, node, c, io, new_here, old_here, f, rho, u);
     // values as float32, as the reference's tensors hold them
+    float ff[L::Q], uf[3];
 #pragma unroll
-    for (int k = 0; k < L::Q; ++k) {
-      const double x = (double)(float)f[k];
-      v[0] = fmin(v[0], x);
-      v[1] = fmax(v[1], x);
-      v[2] += x;
-    }
+    for (int k = 0; k < L::Q; ++k) ff[k] = (float)f[k];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const double x = (double)(float)u[d];
-      v[3] = fmin(v[3], x);
-      v[4] = fmax(v[4], x);
-      v[5] += x;
-    }
-    const double r = (double)(float)rho;
-    v[6] = fmin(v[6], r);
-    v[7] = fmax(v[7], r);
-    v[8] += r;
+    for (int d = 0; d < 3; ++d) uf[d] = (float)u[d];
+    stats_add<L::Q>(v, ff, uf, (float)rho);
   }
-  __shared__ double red[kStats][kBlock];
-#pragma unroll
-  for (int i = 0; i < kStats; ++i) red[i][threadIdx.x] = v[i];
-  __syncthreads();
-  for (int s = kBlock / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-#pragma unroll
-      for (int i = 0; i < kStats; i += 3) {
-        red[i][threadIdx.x] = fmin(red[i][threadIdx.x], red[i][threadIdx.x + s]);
-        red[i + 1][threadIdx.x] = fmax(red[i + 1][threadIdx.x], red[i + 1][threadIdx.x + s]);
-        red[i + 2][threadIdx.x] += red[i + 2][threadIdx.x + s];
-      }
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x < kStats) part[(int64_t)blockIdx.x * kStats + threadIdx.x] = red[threadIdx.x][0];
-}
-
-__global__ void k_stats_final(const double* __restrict__ part, int nblocks, double count_f,
-                              double count_u, double count_r, double* __restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double v[kStats] = {INFINITY, -INFINITY, 0.0, INFINITY, -INFINITY, 0.0, INFINITY, -INFINITY, 0.0};
-  for (int b = 0; b < nblocks; ++b) {
-    for (int i = 0; i < kStats; i += 3) {
-      v[i] = fmin(v[i], part[b * kStats + i]);
-      v[i + 1] = fmax(v[i + 1], part[b * kStats + i + 1]);
-      v[i + 2] += part[b * kStats + i + 2];
-    }
-  }
-  // out: f min, mean, max, u min, mean, max, rho min, mean, max
-  out[0] = v[0];
-  out[1] = v[2] / count_f;
-  out[2] = v[1];
-  out[3] = v[3];
-  out[4] = v[5] / count_u;
-  out[5] = v[4];
-  out[6] = v[6];
-  out[7] = v[8] / count_r;
-  out[8] = v[7];
+  stats_block_reduce(v, part + (int64_t)blockIdx.x * kStats);
 }
 
 }  // namespace funlbm
@@ -132,7 +82,7 @@ int funlbm_flow_stats(const funlbm_lattice* lat, const void* state_src, const fl
     });
   });
   if (rc) return rc;
-  hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(64), 0, s, partials, blocks, (double)p.n * lat->q,
+  hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(kBlock), 0, s, partials, blocks, (double)p.n * lat->q,
                      (double)p.n * 3, (double)p.n, out);
   return check_launch("flow_stats");
 }

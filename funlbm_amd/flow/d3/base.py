@@ -68,6 +68,10 @@ class FlowD3(Flow):
         # fused steps taken since init: the stochastic rounding's dither of a
         # step is drawn from step_seed(seed, steps_done)
         self.steps_done = 0
+        # set by the run loop before a step whose statistics it will log: the
+        # fused step then produces them in the same pass (stats())
+        self.stats_next = False
+        self._stats_valid = False
         self.uzero_mask = _uzero_mask(getattr(config, "uzero_faces", None))
         bf = getattr(config, "body_force", None)
         self.body_force = None if bf is None else [float(v) for v in bf]
@@ -306,11 +310,25 @@ class FlowD3(Flow):
         )
         self._tiles = (tiles, ctl) if use_tiles else (None, None)
         self.lat.seed = nat.step_seed(self.seed, self.steps_done)
-        nat.call(
-            "funlbm_fused_step", self.lat, nat.ptr(self._state[a]), nat.ptr(self._state[b]),
-            fol_old, fol_new, nat.ptr(self._tiles[0]), nat.ptr(self._tiles[1]),
-            nat.ptr(self._rho_face[a]), nat.ptr(self._rho_face[b]), self._stream(),
-        )
+        if self.stats_next:
+            nb = -(-self.n // 256)
+            nb += min(nb, 256)  # and the first stage of the final reduction
+            if getattr(self, "_fstat_part", None) is None or self._fstat_part.numel() < 9 * nb:
+                self._fstat_part = torch.empty(9 * nb, device=self.device, dtype=torch.float64)
+            self._ensure_stat_out()
+            nat.call(
+                "funlbm_fused_step_stats", self.lat, nat.ptr(self._state[a]), nat.ptr(self._state[b]),
+                fol_old, fol_new, nat.ptr(self._tiles[0]), nat.ptr(self._tiles[1]),
+                nat.ptr(self._rho_face[a]), nat.ptr(self._rho_face[b]), nat.ptr(self._fstat_part), nb,
+                nat.ptr(self._stat_out), self._stream(),
+            )
+        else:
+            nat.call(
+                "funlbm_fused_step", self.lat, nat.ptr(self._state[a]), nat.ptr(self._state[b]),
+                fol_old, fol_new, nat.ptr(self._tiles[0]), nat.ptr(self._tiles[1]),
+                nat.ptr(self._rho_face[a]), nat.ptr(self._rho_face[b]), self._stream(),
+            )
+        self._stats_valid = bool(self.stats_next)
         self._cur = b
         self.steps_done += 1
         self.publish_fol(fol_new_written, tiled=use_tiles)
@@ -348,6 +366,7 @@ class FlowD3(Flow):
         v = torch.as_tensor(value, device=self.device, dtype=_F32).reshape(*self.nodes, self.q)
         self._f.copy_(v.permute(3, 0, 1, 2).reshape(self.q, self.n))
         self._pristine = False
+        self._stats_valid = False
 
     @property
     def u(self):
@@ -373,6 +392,7 @@ class FlowD3(Flow):
         self._fol_live[self._fc] = True
         self._fol_tiled[self._fc] = False
         self._pristine = False
+        self._stats_valid = False
         return self._view(self._fol[self._fc], 3)
 
     def fol_array(self):
@@ -388,6 +408,7 @@ class FlowD3(Flow):
         self._fol_live[self._fc] = True
         self._fol_tiled[self._fc] = False
         self._pristine = False
+        self._stats_valid = False
 
     @property
     def feq(self):
@@ -426,9 +447,11 @@ class FlowD3(Flow):
         device tensor.  After fused steps it is one read of the state
         (funlbm_flow_stats) instead of materialising the fields."""
         if self._mode == "fused" and not self._mat_valid:
+            if self._stats_valid:  # produced by the step itself
+                return self._stat_out
             if getattr(self, "_stat_part", None) is None:
                 self._stat_part = torch.empty(9 * 2048, device=self.device, dtype=torch.float64)
-                self._stat_out = torch.empty(9, device=self.device, dtype=torch.float64)
+            self._ensure_stat_out()
             a = 1 - self._cur
             nat.call(
                 "funlbm_flow_stats", self.lat, nat.ptr(self._state[a]), self._fol_ptr(1 - self._fc),
@@ -441,6 +464,10 @@ class FlowD3(Flow):
         for t in (self.f, self.u, self.rou):
             vals += [t.min().double(), t.double().mean(), t.max().double()]
         return torch.stack(vals)
+
+    def _ensure_stat_out(self):
+        if getattr(self, "_stat_out", None) is None:
+            self._stat_out = torch.empty(9, device=self.device, dtype=torch.float64)
 
     def to_json(self):
         """min/mean/max of f, u, rho (src/funlbm/flow/base.py:183-206)."""

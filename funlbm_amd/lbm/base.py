@@ -24,6 +24,11 @@ class Config(BaseConfig):
                  file.cache_dir/checkpoint (default 0: none; the reference
                  writes every file.per_steps = 100 steps, 1.3 GB each for a
                  256^3 D3Q19 lattice)
+      async_log  run() copies its per-step records back without waiting for
+                 the GPU and emits them up to 8 steps late, in order
+                 (default true; false waits for each record, as the
+                 reference); the weight-sum failure stops the run when its
+                 record is emitted
     The reference's `checkpoint` key (a file to resume from) is honoured.
     """
 
@@ -41,6 +46,7 @@ class Config(BaseConfig):
         self.log_every: int = 1
         self.checkpoint_every: int = 0
         self.db_path = None
+        self.async_log: bool = True
 
     def _from_json(self, config_json: dict, *args, **kwargs) -> "Config":
         from ..particle import ParticleConfig
@@ -60,6 +66,8 @@ class Config(BaseConfig):
         self.log_every = 1 if le is None else int(le)
         self.checkpoint_every = int(deep_get(config_json, "checkpoint_every") or 0)
         self.db_path = deep_get(config_json, "db_path") or self.db_path
+        al = deep_get(config_json, "async_log")
+        self.async_log = True if al is None else bool(al)
         return self
 
 
@@ -88,23 +96,40 @@ class LBMBase(Worker):
         self.run_status = True
         self.history = []
         self.store = None
+        self._steplog = None
         if self.config.db_path:
             from .store import StepStore
 
             self.store = StepStore(self.config.db_path)
 
     def run(self, max_steps: int = 1000000, *args, **kwargs) -> None:
-        """src/funlbm/lbm/base.py:86-103."""
+        """src/funlbm/lbm/base.py:86-103.  A logged step asks the flow for
+        its statistics within the step (no second pass over the state)."""
         self.init()
         total_steps = min(max_steps, self.config.max_step)
-        for _ in range(total_steps):
-            self.run_step(step=self.step)
-            self._log_step_info(self.step)
-            if self.run_status is False:
-                break
-            if self.step >= total_steps:
-                break
-            self.step += 1
+        every = self.config.log_every
+        if self.config.async_log and every and getattr(self.flow, "device", None) is not None \
+                and str(self.flow.device).startswith("cuda"):
+            from .steplog import StepLog
+
+            self._steplog = StepLog(self)
+        try:
+            for _ in range(total_steps):
+                if hasattr(self.flow, "stats_next"):
+                    self.flow.stats_next = bool(every) and self.step % every == 0
+                self.run_step(step=self.step)
+                self._log_step_info(self.step)
+                if self.run_status is False:
+                    break
+                if self.step >= total_steps:
+                    break
+                self.step += 1
+        finally:
+            if hasattr(self.flow, "stats_next"):
+                self.flow.stats_next = False
+            if self._steplog is not None:
+                self._steplog.drain(wait=True)
+                self._steplog = None
 
     def _log_step_info(self, *args, **kwargs) -> None:
         """Per-step statistics (src/funlbm/lbm/base.py:105-144) at the
@@ -112,17 +137,27 @@ class LBMBase(Worker):
         every = self.config.log_every
         if not every or self.step % every:
             return
+        if self._steplog is not None:
+            self._steplog.record(self.step)
+            return
         self._check_run_status()
-        rec = {"step": self.step, "flow": self.flow.to_json()}
-        rec["particles"] = [p.to_json() for p in self.particle_swarm.particles]
-        self.history.append(rec)
+        flow = self.flow.to_json()
+        particles = [p.to_json() for p in self.particle_swarm.particles]
+        self._emit_record(self.step, flow, particles)
+
+    def _emit_record(self, step: int, flow: dict, particles: list) -> None:
+        """History entry, SQLite rows and the reference's log line of one step."""
+        from ..particle.base.base import format_particle
+
+        self.history.append({"step": step, "flow": flow, "particles": particles})
         if self.store is not None:
-            self.store.put(self.step, rec["flow"], rec["particles"])
-        fl = rec["flow"]
-        info = [f"step={self.step:6d}"] + [
-            f"{k}=" + ",".join(f"{v:.6f}" for v in fl[k]) for k in ("f", "u", "rho")
+            self.store.put(step, flow, particles)
+        if not logger.isEnabledFor(logging.INFO):
+            return
+        info = [f"step={step:6d}"] + [
+            f"{k}=" + ",".join(f"{v:.6f}" for v in flow[k]) for k in ("f", "u", "rho")
         ]
-        info += [p.to_str(self.step) for p in self.particle_swarm.particles]
+        info += [format_particle(d) for d in particles]
         logger.info("\t".join(info))
 
     def _check_run_status(self):

@@ -128,18 +128,7 @@ class Particle:
         self.swarm.pose(only=self.index)
 
     def to_str(self, step=0, *args, **kwargs):
-        def fmt(v):
-            return ",".join(f"{i:.6f}" for i in v)
-
-        lf = self.lF
-        res = f"m={self.mass:.2f}"
-        res += "\tcu=" + fmt(self.cu.tolist())
-        res += "\tcx=" + fmt(self.cx.tolist())
-        res += "\tcf=" + fmt(self.cF.tolist())
-        res += "\tlF=" + fmt([float(lf.min()), float(lf.mean()), float(lf.max())])
-        res += "\tcw=" + fmt(self.cw.tolist())
-        res += "\tr=" + self.coord.to_str()
-        return res
+        return format_particle(self.to_json(step))
 
     def to_json(self, step=0, *args, **kwargs):
         lf = self.lF
@@ -152,3 +141,21 @@ class Particle:
             "cw": self.cw.tolist(),
             "coord": self.coord.to_json(),
         }
+
+
+def format_particle(d: dict) -> str:
+    """The reference's per-particle log text (src/funlbm/particle/base/base.py:161-171)
+    from a to_json record."""
+
+    def fmt(v):
+        return ",".join(f"{i:.6f}" for i in v)
+
+    res = f"m={d['m']:.2f}"
+    res += "\tcu=" + fmt(d["cu"])
+    res += "\tcx=" + fmt(d["cx"])
+    res += "\tcf=" + fmt(d["cf"])
+    res += "\tlF=" + fmt(d["lF"])
+    res += "\tcw=" + fmt(d["cw"])
+    c = d["coord"]
+    res += "\tr=center=" + fmt(c["center"]) + "\tangle=" + fmt(c["angle"])
+    return res

@@ -172,6 +172,20 @@ int funlbm_fused_step(const funlbm_lattice* lat, const void* state_src, void* st
                       const float* fol_old, const float* fol_new, const uint32_t* fol_tiles,
                       const int32_t* ctl, const float* rho_prev, float* rho_next, void* stream);
 
+/* funlbm_fused_step that also produces the statistics funlbm_flow_stats
+ * would read from state_src afterwards (min, mean, max of the reference's f,
+ * u and rho after this step's stream and moments, the per-step log of
+ * src/funlbm/lbm/base.py:105-144), in the same pass over the state.
+ * partials: device double[9 * n_partials], n_partials >= ceil(nodes / 256)
+ * + min(ceil(nodes / 256), 256);
+ * out: device double[9] as funlbm_flow_stats.  Deterministic: fixed-order
+ * reductions.  No reference counterpart (the reference reduces the fields
+ * after the step). */
+int funlbm_fused_step_stats(const funlbm_lattice* lat, const void* state_src, void* state_dst,
+                            const float* fol_old, const float* fol_new, const uint32_t* fol_tiles,
+                            const int32_t* ctl, const float* rho_prev, float* rho_next, double* partials,
+                            int32_t n_partials, double* out, void* stream);
+
 /* The same pass as funlbm_fused_step without the collision: writes the
  * reference's f, u, rho and p (float) after the step.  Any output may be NULL. */
 int funlbm_materialize(const funlbm_lattice* lat, const void* state_src, const float* fol_old,
@@ -304,7 +318,8 @@ int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, in
  * step, computed from the state without materialising the fields (the
  * statistics of src/funlbm/flow/base.py:183-206 that the run loop logs every
  * step, src/funlbm/lbm/base.py:105-144).  out: double[9] = f min, mean, max,
- * u min, mean, max, rho min, mean, max.  partials: device scratch of
+ * u min, mean, max, rho min, mean, max; the mean of f is the mean of rho
+ * over q (a node's populations sum to its density).  partials: device scratch of
  * 9 * n_partials doubles (one entry per block; n_partials >= 1 bounds the
  * grid).  Deterministic: fixed-order reductions. */
 int funlbm_flow_stats(const funlbm_lattice* lat, const void* state_src, const float* fol_old,

@@ -186,7 +186,36 @@ def test_flow_stats_from_state(storage):
         want += [t.min().item(), t.double().mean().item(), t.max().item()]
     want = np.array(want)
     assert np.array_equal(fast[[0, 2, 3, 5, 6, 8]], want[[0, 2, 3, 5, 6, 8]])
-    assert np.allclose(fast[[1, 4, 7]], want[[1, 4, 7]], rtol=1e-9, atol=1e-15)
+    assert np.allclose(fast[[4, 7]], want[[4, 7]], rtol=1e-9, atol=1e-15)
+    # the mean of f is the mean density over Q (a node's populations sum to
+    # its density up to the float32 rounding of that sum)
+    assert abs(fast[1] - want[7] / lbm.flow.q) <= 1e-12 * abs(fast[1])
+    assert abs(fast[1] - want[1]) <= 1e-6 * abs(want[1])
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+@pytest.mark.parametrize("q", [19, 27])
+def test_step_statistics_equal_separate_pass(q, storage):
+    """The statistics that a logged fused step reduces in its own pass
+    (funlbm_fused_step_stats) equal those of the separate pass over the state
+    (funlbm_flow_stats): min and max exactly, means to 1e-12 (the two group
+    the nodes differently); the state the step writes is the plain step's."""
+    size = (9, 11, 13)
+    bcs = BC_SETS["channel"]
+    nodes = tuple(s + 1 for s in size)
+    f0 = random_state(q, nodes, seed=5)
+    a = run_solver(q, size, 0.05, bcs, 5, f0=f0, storage=storage, body_force=[0.0, 1e-5, 0.0])
+    b = run_solver(q, size, 0.05, bcs, 5, f0=f0, storage=storage, body_force=[0.0, 1e-5, 0.0])
+    a.flow.stats_next = True
+    for lbm in (a, b):
+        lbm.step = 6
+        lbm.run_step()
+    fused = a.flow.stats().cpu().numpy().copy()
+    a.flow._stats_valid = False
+    sep = a.flow.stats().cpu().numpy()
+    assert np.array_equal(fused[[0, 2, 3, 5, 6, 8]], sep[[0, 2, 3, 5, 6, 8]])
+    assert np.allclose(fused[[1, 4, 7]], sep[[1, 4, 7]], rtol=1e-12, atol=1e-15)
+    assert torch.equal(a.flow.f, b.flow.f)
 
 
 @pytest.mark.parametrize("size", [(11, 9, 13), (9, 10, 15)])

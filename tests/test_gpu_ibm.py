@@ -153,3 +153,47 @@ def test_nonfinite_markers_stay_in_bounds():
     outside = fol.clone()
     outside[:4, :4, :4] = 0
     assert torch.count_nonzero(outside) == 0
+
+
+def test_async_step_records_equal_synchronous_ones():
+    """run() with records copied back asynchronously logs the same history as
+    with each record waited for: flow statistics and particle rows exactly,
+    the marker-force mean to float32 rounding."""
+    runs = []
+    for async_log in (False, True):
+        lbm = make_case(particles=SPHERE)
+        lbm.config.max_step = 12
+        lbm.config.log_every = 3
+        lbm.config.async_log = async_log
+        lbm.run()
+        runs.append(lbm.history)
+    sync, asyn = runs
+    assert [r["step"] for r in sync] == [r["step"] for r in asyn] == [3, 6, 9, 12]
+    for rs, ra in zip(sync, asyn):
+        assert rs["flow"] == ra["flow"]
+        for ps, pa in zip(rs["particles"], ra["particles"]):
+            for key in ("m", "cu", "cx", "cf", "cw", "coord"):
+                assert ps[key] == pa[key], key
+            assert ps["lF"][0] == pa["lF"][0] and ps["lF"][2] == pa["lF"][2]
+            assert abs(ps["lF"][1] - pa["lF"][1]) <= 1e-6 * max(1e-30, abs(ps["lF"][0]), abs(ps["lF"][2]))
+
+
+def test_async_records_stop_the_run_on_failure():
+    """The weight-sum failure flag travels with the asynchronous records:
+    run() stops within the record lag of the failing step."""
+    lbm = make_case(particles=SPHERE)
+    lbm.config.max_step = 130
+    lbm.config.log_every = 1
+    lbm.init()
+    # push a marker outside the box before step 101's check is armed
+    original = lbm.run_step
+    def run_step(*a, **k):
+        if lbm.step == 101:
+            lbm.particle_swarm.lx[0] = torch.tensor([40.0, 40.0, 40.0], device=lbm.device)
+        return original(*a, **k)
+    lbm.run_step = run_step
+    lbm.init = lambda *a, **k: None  # keep the state prepared above
+    lbm.step = 1
+    lbm.run()
+    assert lbm.run_status is False
+    assert 101 <= lbm.step <= 101 + 9
