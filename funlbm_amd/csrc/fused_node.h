@@ -209,9 +209,11 @@ __device__ __forceinline__ void collide_store_dev(const KParams& p, int64_t node
   }
 }
 
-// The fused step of a 16-bit state in deviation arithmetic (step_node_dev,
-// collide_store_dev): the populations stay scaled deviations from the load to
-// the store.
+// The fused step of a 16-bit state in deviation arithmetic: the populations
+// stay scaled deviations from the load to the store.  The collision and store
+// are written out here as in collide_store_dev: with the call, the compiler
+// merged the multi-lattice kernel's pull paths differently and that kernel
+// ran 40 % slower on the MI355X.
 template <class L, class S, bool FORCE>
 __device__ __forceinline__ void fused_node_dev(const KParams& p, int64_t node, const FusedIO<S>& io,
                                                bool new_here, bool old_here, uint32_t seed_mix) {
@@ -219,7 +221,36 @@ __device__ __forceinline__ void fused_node_dev(const KParams& p, int64_t node, c
   float g[L::Q];
   float rho, u[3], drho;
   step_node_dev<L, S, FORCE>(p, node, c, io, new_here, old_here, g, rho, u, &drho);
-  collide_store_dev<L, S>(p, node, c, io, g, rho, drho, u, seed_mix);
+  collide_dev<L>(p, g, u, rho, drho);
+  if (p.has_moving && io.rho_next) {
+    bool mf = false;
+#pragma unroll
+    for (int face = 0; face < 6; ++face)
+      mf |= (p.bc[face] == FUNLBM_BC_WALL_WITH_SPEED) && on_face(p, c, face);
+    if (mf) io.rho_next[node] = rho;
+  }
+  using R = typename S::raw;
+  constexpr int Q = L::Q;
+  const int ts = p.ts;
+  const uint32_t tmask = (1u << ts) - 1u;
+  const int tn = (__builtin_amdgcn_readfirstlane((int)node) & ~63) >> ts;
+  const uint32_t lb = ((uint32_t)node & tmask) * (uint32_t)sizeof(R);
+  if (p.sr) {
+    const uint32_t h = mix32((uint32_t)node ^ seed_mix);
+    uint32_t pair = 0;
+    static_for<0, Q>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      char* base = (char*)io.dst + ((int64_t)Q + (int64_t)tn * Q + K) * ((int64_t)1 << ts) * (int64_t)sizeof(R);
+      pair = S::template cvt_sr_half<(K & 1) != 0>(pair, g[K], __umul24(h, dither_mult(K)));
+      __builtin_nontemporal_store((R)(K & 1 ? pair >> 16 : pair), (R*)(base + lb));
+    });
+  } else {
+    static_for<0, Q>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      char* base = (char*)io.dst + ((int64_t)Q + (int64_t)tn * Q + K) * ((int64_t)1 << ts) * (int64_t)sizeof(R);
+      __builtin_nontemporal_store(S::from_float(g[K]), (R*)(base + lb));
+    });
+  }
 }
 
 // Statistics of the reference's f, u and rho of one node (the values as
