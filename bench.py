@@ -402,7 +402,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
-    ap.add_argument("--serve-steps", default="100,2000", help="min,max steps of a request")
+    ap.add_argument("--serve-steps", default="1000,20000", help="min,max steps of a request")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-f16", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
