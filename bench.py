@@ -412,8 +412,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(
-            "bench.py measures one GPU: this build runs one lattice per GPU and no multi-GPU path "
-            "(see DESIGN.md for how the lattice would shard); run with --gpus 1",
+            "bench.py measures one GPU: a lattice runs on one GPU and there is no multi-GPU "
+            "path; run with --gpus 1",
             file=sys.stderr,
         )
         sys.exit(2)
