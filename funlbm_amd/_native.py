@@ -22,7 +22,7 @@ BC_WALL = 1200
 BC_WALL_WITH_SPEED = 1201
 BC_NON_EQUILIBRIUM = 1400
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 ROUND_NEAREST = 0
 ROUND_STOCHASTIC = 1
@@ -98,6 +98,11 @@ SIGNATURES = {
         c_int32,
         [LATP, P, P, P, P, P, c_int32, P, P, P, c_float, c_int32, P, P, P, P],
     ),
+    "funlbm_ibm_interp_boxed": (
+        c_int32,
+        [LATP, P, P, P, c_int32, P, P, c_int32, P, P, c_int32, P, P, c_int32, c_float, c_int32, P,
+         P, P, P],
+    ),
     "funlbm_wall_spring": (c_int32, [LATP, c_int32, P, P, P, c_float, P]),
     "funlbm_particle_forces": (c_int32, [c_int32, P, P, P, P, P, ctypes.c_double, ctypes.c_double, P]),
     "funlbm_particle_pose": (c_int32, [c_int32, P, P, P, P, c_int32, P]),
@@ -106,7 +111,7 @@ SIGNATURES = {
     "funlbm_flow_stats": (c_int32, [LATP, P, P, P, P, P, P, P, c_int32, P, P]),
     "funlbm_case_bytes": (c_int32, []),
     "funlbm_case_pack": (c_int32, [LATP, c_int64, c_int64, c_int32, P]),
-    "funlbm_batch_step": (c_int32, [c_int32] + [P] * 30 + [c_int32] * 4 + [P]),
+    "funlbm_batch_step": (c_int32, [c_int32] + [P] * 30 + [c_int32] * 4 + [P, P, P, c_int32, P]),
 }
 
 _lib = None

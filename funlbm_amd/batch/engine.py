@@ -20,11 +20,13 @@ import numpy as np
 import torch
 
 from .. import _native as nat
+from ..particle import boxes
 from ..particle.markers import surface_markers
 from .alloc import RangeAllocator
 
 TILE = 256
 H_TILES, H_MARKERS, H_PARTICLES, H_CASES, H_STAMP = 0, 1, 2, 3, 4
+H_CHUNKS, H_WORDS = 8, 9
 
 
 @dataclasses.dataclass
@@ -126,14 +128,19 @@ class _Slot:
     p_off: int
     p_count: int
     plane: int
+    boxes: list  # (cache offset, R) per particle of the case
 
 
 class BatchEngine:
     """Pools for up to `max_cases` cases holding at most `max_nodes` nodes,
-    `max_markers` markers and `max_particles` particles in total."""
+    `max_markers` markers and `max_particles` particles in total.  The
+    particles' moment boxes share a cache of `max_box_nodes` nodes (default
+    32 per marker slot); a particle whose box does not fit goes without one
+    (its markers pull per node, with the same results)."""
 
     def __init__(self, storage="f16", max_nodes=1 << 24, max_markers=1 << 16, max_particles=256,
-                 max_cases=256, max_q=27, traj_cap=64, dev_scale=1024.0, device=None, rounding=None):
+                 max_cases=256, max_q=27, traj_cap=64, dev_scale=1024.0, device=None, rounding=None,
+                 max_box_nodes=None):
         from ..base.work import device_detect
 
         self.device = device_detect(device)
@@ -158,6 +165,10 @@ class BatchEngine:
         self.tiles = RangeAllocator(tile_cap)
         self.markers = RangeAllocator(int(max_markers))
         self.parts = RangeAllocator(int(max_particles))
+        self.max_particles = int(max_particles)
+        box_cap = int(max_box_nodes) if max_box_nodes is not None else 32 * int(max_markers)
+        self.box_cap = min(box_cap, (1 << 31) - 1)
+        self.box_alloc = RangeAllocator(self.box_cap)
         self.free_cases = list(range(self.max_cases - 1, -1, -1))
         dt = nat.STORAGE_DTYPES[self.storage]
         i32, f32 = torch.int32, torch.float32
@@ -187,7 +198,11 @@ class BatchEngine:
         self.case_marker_begin = z(self.max_cases, i32)
         self.tile_case = torch.full((tile_cap,), -1, device=dev, dtype=i32)
         self.tile_local = z(tile_cap, i32)
-        self.hdr = z(8, i32)
+        self.hdr = z(H_WORDS, i32)
+        self.box_cache = torch.empty(4 * max(self.box_cap, 1), device=dev, dtype=f32)
+        self.particle_box = z(3 * int(max_particles), i32)
+        self.chunk_particle = torch.full((self.box_cap // boxes.CHUNK + int(max_particles),), -1,
+                                         device=dev, dtype=i32)
         self.traj = z(int(max_particles) * self.traj_cap * 4, f32)
         self.traj_cap_t = torch.tensor([self.traj_cap], device=dev, dtype=i32)
         self.slots = {}
@@ -234,6 +249,12 @@ class BatchEngine:
         m_off = self.markers.alloc(m_count) if m_count else 0
         p_off = self.parts.alloc(len(spec.particles)) if spec.particles else 0
         cid = self.free_cases.pop()
+        # moment boxes: a particle whose box does not fit the cache goes without
+        pboxes = []
+        for lag, *_ in infos:
+            R = boxes.box_radius(lag)
+            off = self.box_alloc.alloc(boxes.box_nodes(R)) if R >= 0 else None
+            pboxes.append((off, R) if off is not None else (0, -1))
         lat = lattice_for(spec, self.storage, self.dev_scale, self.rounding)
         desc = (ctypes.c_uint8 * self.case_bytes)()
         nat.call("funlbm_case_pack", lat, f_off, node_off, tile_off, desc)
@@ -289,7 +310,7 @@ class BatchEngine:
         self.origin_ok[cid] = dyn[4]
         self.case_marker_begin[cid] = dyn[5]
         self.slots[cid] = _Slot(spec, lat, f_off, node_off, tile_off, n_tiles, m_off, m_count, p_off,
-                                len(spec.particles), plane)
+                                len(spec.particles), plane, pboxes)
         self._dirty = True
         return cid
 
@@ -302,6 +323,9 @@ class BatchEngine:
         if s.p_count:
             self.particle_case[s.p_off : s.p_off + s.p_count] = -1
             self.parts.free(s.p_off, s.p_count)
+        for off, R in s.boxes:
+            if R >= 0:
+                self.box_alloc.free(off, boxes.box_nodes(R))
         self.elems.free(s.f_off, s.spec.q * s.plane)
         self.nodes.free(s.node_off, s.spec.n)
         self.tiles.free(s.tile_off, s.n_tiles)
@@ -326,8 +350,19 @@ class BatchEngine:
         if nt:
             self.tile_case[:nt] = torch.from_numpy(np.concatenate(tc)).to(self.device)
             self.tile_local[:nt] = torch.from_numpy(np.concatenate(tl)).to(self.device)
+        # moment boxes over particle slots, and the chunk list of the box kernel
+        radii = np.full(self.max_particles, -1, dtype=np.int64)
+        offs = np.zeros(self.max_particles, dtype=np.int64)
+        for cid, s in sorted(self.slots.items()):
+            for k, (off, R) in enumerate(s.boxes):
+                radii[s.p_off + k], offs[s.p_off + k] = R, off
+        rows, chunks, _ = boxes.layout(radii, offs)
+        self.particle_box.copy_(torch.from_numpy(rows.reshape(-1)))
+        nch = len(chunks)
+        if nch:
+            self.chunk_particle[:nch] = torch.from_numpy(chunks).to(self.device)
         stamp = int(self.hdr[H_STAMP].item())
-        hdr = [nt, self.markers.high_water, self.parts.high_water, self.max_cases, stamp] + ends[:3]
+        hdr = [nt, self.markers.high_water, self.parts.high_water, self.max_cases, stamp] + ends[:3] + [nch]
         self.hdr.copy_(torch.tensor(hdr, dtype=torch.int32))
         self._with_ibm = 1 if self.markers.high_water > 0 else 0
         self._dirty = False
@@ -344,7 +379,8 @@ class BatchEngine:
             P(self.fol[0]), P(self.fol[1]), P(self.owner), P(self.fol_tiles), P(self.origin[0]),
             P(self.origin[1]), P(self.origin_ok), P(self.lag), P(self.lx), P(self.lu), P(self.lF),
             P(self.lm), P(self.pstate), P(self.traj), P(self.traj_cap_t), int(g["tiles"]),
-            int(g["markers"]), int(g["particles"]), 1, nat.stream_handle(self.device),
+            int(g["markers"]), int(g["particles"]), 1, P(self.particle_box), P(self.chunk_particle),
+            P(self.box_cache), self.box_cap, nat.stream_handle(self.device),
         )
 
     def step(self, n: int = 1):

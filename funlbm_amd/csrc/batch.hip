@@ -29,7 +29,8 @@ enum : int {
   H_STAMP = 4,      // stamp of the last completed spreading
   H_QEND = 5,       // [5], [6], [7]: end of the D3Q13, D3Q15, D3Q19 tiles (the list is
                     // ordered by lattice; D3Q27 tiles run to hdr[H_TILES])
-  H_WORDS = 8
+  H_CHUNKS = 8,     // 256-node chunks of the particles' moment boxes
+  H_WORDS = 9
 };
 
 struct CaseDesc {
@@ -113,11 +114,13 @@ __global__ __launch_bounds__(kIbmBlock) void k_batch_interp(
     const typename S::elem* __restrict__ pool_a, const typename S::elem* __restrict__ pool_b,
     const float* __restrict__ fol_a, const float* __restrict__ fol_b,
     const float* __restrict__ lx, const double* __restrict__ ps, float* __restrict__ lu,
-    float* __restrict__ lF) {
+    float* __restrict__ lF, const int* __restrict__ pbox, const float4* __restrict__ cache, int cap) {
   const int nm = hdr[H_MARKERS];
   const int lane = threadIdx.x & (kWave - 1);
   const int waves = gridDim.x * (kIbmBlock / kWave);
-  for (int m = blockIdx.x * (kIbmBlock / kWave) + threadIdx.x / kWave; m < nm; m += waves) {
+  // consecutive markers (one particle's, sharing box nodes) on one XCD's L2
+  const int b = (int)xcd_block(blockIdx.x, gridDim.x);
+  for (int m = b * (kIbmBlock / kWave) + threadIdx.x / kWave; m < nm; m += waves) {
     const int c = marker_case[m];
     if (c < 0 || dyn.left[c] <= 0) continue;
     const CaseDesc& cd = cases[c];
@@ -127,8 +130,37 @@ __global__ __launch_bounds__(kIbmBlock) void k_batch_interp(
     const bool check = dyn.done[c] + 1 > 100;
     with_lattice(cd.q, [&](auto l) {
       using L = decltype(l);
-      interp_marker<L, S, true>(cd.p, m, lane, st, fo, nullptr, nullptr, nullptr, lx,
-                                pst(ps, marker_particle[m]), 1.0f, check, lu, lF, &dyn.flag[c]);
+      const int pi = marker_particle[m];
+      interp_marker<L, S, true>(cd.p, m, lane, st, fo, nullptr, nullptr, nullptr, lx, pst(ps, pi),
+                                1.0f, check, lu, lF, &dyn.flag[c], cache ? pbox + 3 * pi : nullptr,
+                                cache, cap);
+    });
+  }
+}
+
+// Moment boxes of the active cases' particles (box_node), before
+// k_batch_interp reads them.  chunk_particle names each chunk's particle.
+template <class S>
+__global__ __launch_bounds__(kBoxChunk) void k_batch_box_moments(
+    const CaseDesc* __restrict__ cases, const int* __restrict__ hdr, BatchDyn dyn,
+    const int* __restrict__ particle_case, const typename S::elem* __restrict__ pool_a,
+    const typename S::elem* __restrict__ pool_b, const float* __restrict__ fol_a,
+    const float* __restrict__ fol_b, const double* __restrict__ ps, const int* __restrict__ pbox,
+    const int* __restrict__ chunk_particle, float4* __restrict__ cache, int cap) {
+  const int nch = hdr[H_CHUNKS];
+  const int np = hdr[H_PARTICLES];
+  for (int ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+    const int pi = chunk_particle[ch];
+    if ((unsigned)pi >= (unsigned)np) continue;
+    const int c = particle_case[pi];
+    if (c < 0 || dyn.left[c] <= 0) continue;
+    const CaseDesc& cd = cases[c];
+    const int par = dyn.par[c];
+    const typename S::elem* st = (par ? pool_b : pool_a) + cd.f_off;
+    const float* fo = (par ? fol_b : fol_a) + 3 * cd.node_off;
+    with_lattice(cd.q, [&](auto l) {
+      using L = decltype(l);
+      box_node<L, S>(cd.p, st, fo, nullptr, pst(ps, pi), pbox + 3 * pi, ch, cache, cap);
     });
   }
 }
@@ -292,7 +324,9 @@ int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, in
                       int32_t* origin_b, int32_t* origin_ok, const float* lag, float* lx, float* lu,
                       float* lF, const float* lm, double* pstate, float* traj,
                       const int32_t* traj_cap, int32_t grid_tiles, int32_t grid_markers,
-                      int32_t grid_particles, int32_t with_ibm, void* stream) {
+                      int32_t grid_particles, int32_t with_ibm, const int32_t* particle_box,
+                      const int32_t* chunk_particle, float* box_cache, int32_t cache_nodes,
+                      void* stream) {
   if (!cases || !hdr || !case_par || !case_left || !case_done || !case_flag || !tile_case ||
       !tile_local || !pool_a || !pool_b || !fol_a || !fol_b || !fol_tiles)
     return set_error(FUNLBM_EINVAL, "batch_step: NULL array");
@@ -302,6 +336,11 @@ int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, in
                    !particle_markers || !owner || !origin_a || !origin_b || !origin_ok || !lag ||
                    !lx || !lu || !lF || !lm || !pstate))
     return set_error(FUNLBM_EINVAL, "batch_step: NULL particle array");
+  if (with_ibm && box_cache && (!particle_box || !chunk_particle || cache_nodes < 0))
+    return set_error(FUNLBM_EINVAL, "batch_step: box cache without its tables");
+  if (box_cache && ((uintptr_t)box_cache & 15))
+    return set_error(FUNLBM_EINVAL, "batch_step: box cache must be 16-byte aligned");
+  const float4* cache = (with_ibm && box_cache) ? (const float4*)box_cache : nullptr;
   const CaseDesc* cd = (const CaseDesc*)cases;
   int32_t* h = (int32_t*)hdr;
   BatchDyn dyn{case_par, case_left, case_done, case_flag};
@@ -311,9 +350,15 @@ int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, in
     using S = decltype(st);
     using E = typename S::elem;
     if (with_ibm) {
+      if (cache) {
+        hipLaunchKernelGGL((k_batch_box_moments<S>), dim3(grid_markers), dim3(kBoxChunk), 0, s, cd,
+                           hdr, dyn, particle_case, (const E*)pool_a, (const E*)pool_b, fol_a, fol_b,
+                           pstate, particle_box, chunk_particle, (float4*)box_cache, cache_nodes);
+        if ((rc = check_launch("batch_box_moments"))) return rc;
+      }
       hipLaunchKernelGGL((k_batch_interp<S>), dim3(grid_markers), dim3(kIbmBlock), 0, s, cd, hdr, dyn,
                          marker_case, marker_particle, (const E*)pool_a, (const E*)pool_b, fol_a,
-                         fol_b, lx, pstate, lu, lF);
+                         fol_b, lx, pstate, lu, lF, particle_box, cache, cache_nodes);
       if ((rc = check_launch("batch_interp"))) return rc;
       hipLaunchKernelGGL(k_batch_wall, dim3(grid_particles), dim3(kIbmBlock), 0, s, cd, hdr, dyn,
                          particle_case, particle_markers, lx, lF, 1.0f);

@@ -18,12 +18,32 @@ __global__ __launch_bounds__(kIbmBlock) void k_interp(
     const float* __restrict__ rho_face, const float* __restrict__ ufield,
     const float* __restrict__ rhofield, int nm, const float* __restrict__ lx,
     const int* __restrict__ pid, const double* __restrict__ ps, float dt, int check_sum,
-    float* __restrict__ lu, float* __restrict__ lF, int* __restrict__ ctl) {
-  const int m = blockIdx.x * (kIbmBlock / kWave) + threadIdx.x / kWave;
+    float* __restrict__ lu, float* __restrict__ lF, int* __restrict__ ctl,
+    const int* __restrict__ pbox, const float4* __restrict__ cache, int cap) {
+  // consecutive markers (one particle's, sharing box nodes) on one XCD's L2
+  const int b = (int)xcd_block(blockIdx.x, gridDim.x);
+  const int m = b * (kIbmBlock / kWave) + threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (m >= nm) return;
+  const int pi = pid[m];
   interp_marker<L, S, FROM_STATE>(p, m, lane, state, fol_old, rho_face, ufield, rhofield, lx,
-                                  pst(ps, pid[m]), dt, check_sum != 0, lu, lF, &ctl[1]);
+                                  pst(ps, pi), dt, check_sum != 0, lu, lF, &ctl[1],
+                                  cache ? pbox + 3 * pi : nullptr, cache, cap);
+}
+
+// Moment cache of the particles' boxes (box_node), one 256-node chunk per
+// block pass.
+template <class L, class S>
+__global__ __launch_bounds__(kBoxChunk) void k_box_moments(
+    KParams p, const typename S::elem* __restrict__ state, const float* __restrict__ fol_old,
+    const float* __restrict__ rho_face, const double* __restrict__ ps, const int* __restrict__ pbox,
+    const int* __restrict__ chunk_particle, int n_chunks, int np, float4* __restrict__ cache,
+    int cap) {
+  for (int c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+    const int pi = chunk_particle[c];
+    if ((unsigned)pi >= (unsigned)np) continue;
+    box_node<L, S>(p, state, fol_old, rho_face, pst(ps, pi), pbox + 3 * pi, c, cache, cap);
+  }
 }
 
 __global__ __launch_bounds__(kIbmBlock) void k_wall(KParams p, const int* __restrict__ offsets,
@@ -111,11 +131,57 @@ int funlbm_ibm_interp(const funlbm_lattice* lat, const void* state, const float*
       if (from_state)
         hipLaunchKernelGGL((k_interp<L, S, true>), dim3(wave_grid(n_markers)), dim3(kIbmBlock), 0, s, p,
                            (const E*)state, fol_old, rho_face, u, rho, n_markers, lx, marker_particle,
-                           pstate, dt, check_weight_sum, lu, lF, ctl);
+                           pstate, dt, check_weight_sum, lu, lF, ctl, nullptr, nullptr, 0);
       else
         hipLaunchKernelGGL((k_interp<L, S, false>), dim3(wave_grid(n_markers)), dim3(kIbmBlock), 0, s,
                            p, (const E*)state, fol_old, rho_face, u, rho, n_markers, lx,
-                           marker_particle, pstate, dt, check_weight_sum, lu, lF, ctl);
+                           marker_particle, pstate, dt, check_weight_sum, lu, lF, ctl, nullptr,
+                           nullptr, 0);
+      return check_launch("ibm_interp");
+    });
+  });
+}
+
+int funlbm_ibm_interp_boxed(const funlbm_lattice* lat, const void* state, const float* fol_old,
+                            const float* rho_face, int32_t n_markers, const float* lx,
+                            const int32_t* marker_particle, int32_t n_particles,
+                            const double* pstate, const int32_t* particle_box, int32_t n_chunks,
+                            const int32_t* chunk_particle, float* box_cache, int32_t cache_nodes,
+                            float dt, int32_t check_weight_sum, float* lu, float* lF, int32_t* ctl,
+                            void* stream) {
+  KParams p;
+  int rc = make_params(lat, &p);
+  if (rc) return rc;
+  if (n_markers < 0 || n_particles < 0 || n_chunks < 0 || cache_nodes < 0)
+    return set_error(FUNLBM_EINVAL, "ibm_interp_boxed: negative count");
+  if (n_markers == 0) return 0;
+  if ((rc = reject_neq(p, "ibm_interp from a state"))) return rc;
+  if (!state || !lx || !marker_particle || !pstate || !particle_box || !lu || !lF || !ctl ||
+      (n_chunks > 0 && (!chunk_particle || !box_cache)))
+    return set_error(FUNLBM_EINVAL, "ibm_interp_boxed: NULL array");
+  if (box_cache && ((uintptr_t)box_cache & 15))
+    return set_error(FUNLBM_EINVAL, "ibm_interp_boxed: box cache must be 16-byte aligned");
+  if (p.has_moving && !rho_face)
+    return set_error(FUNLBM_EINVAL, "ibm_interp: moving wall needs the face density");
+  if (!(dt > 0.0f)) return set_error(FUNLBM_EINVAL, "ibm_interp: dt must be positive");
+  hipStream_t s = (hipStream_t)stream;
+  const float4* cache = n_chunks > 0 ? (const float4*)box_cache : nullptr;
+  return dispatch_lattice(lat->q, [&](auto l) {
+    using L = decltype(l);
+    return dispatch_storage(lat->storage, [&](auto st) {
+      using S = decltype(st);
+      using E = typename S::elem;
+      if (cache) {
+        hipLaunchKernelGGL((k_box_moments<L, S>), dim3(n_chunks < 65536 ? n_chunks : 65536),
+                           dim3(kBoxChunk), 0, s, p, (const E*)state, fol_old, rho_face, pstate,
+                           particle_box, chunk_particle, n_chunks, n_particles, (float4*)box_cache,
+                           cache_nodes);
+        if ((rc = check_launch("ibm_box_moments"))) return rc;
+      }
+      hipLaunchKernelGGL((k_interp<L, S, true>), dim3(wave_grid(n_markers)), dim3(kIbmBlock), 0, s, p,
+                         (const E*)state, fol_old, rho_face, nullptr, nullptr, n_markers, lx,
+                         marker_particle, pstate, dt, check_weight_sum, lu, lF, ctl, particle_box,
+                         cache, cache_nodes);
       return check_launch("ibm_interp");
     });
   });

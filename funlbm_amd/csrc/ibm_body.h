@@ -93,12 +93,113 @@ __device__ inline const double* pst(const double* ps, int i) {
 }
 
 // ---------------------------------------------------------------------------
+// Moments of node (x, y, z) as the interpolation sees them: the post-stream
+// populations pulled out of the fused state, with FOL_old as the forcing
+// (a 16-bit state in its deviation arithmetic, as its fused step).
+// ---------------------------------------------------------------------------
+template <class L, class S>
+__device__ __forceinline__ void state_node_moments(const KParams& p,
+                                                   const typename S::elem* __restrict__ state,
+                                                   const float* __restrict__ fol_old,
+                                                   const float* __restrict__ rho_face, int x, int y,
+                                                   int z, float& rr, float uu[3]) {
+  using T = typename S::T;
+  const int64_t node = lin(p, x, y, z);
+  const Node c{x, y, z};
+  if constexpr (S::fast) {
+    float g[L::Q];
+    pull_node<L, DevOf<S>, false>(p, state, c, node, rho_face, g);
+    float F[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) F[d] = fol_old ? fol_old[(int64_t)d * p.n + node] + p.g[d] : p.g[d];
+    moments_dev<L>(p, c, g, F, rr, uu);
+  } else {
+    T f[L::Q];
+    pull_node<L, S, false>(p, state, c, node, rho_face, f);
+    T F[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      F[d] = (T)(fol_old ? fol_old[(int64_t)d * p.n + node] + p.g[d] : p.g[d]);
+    T r, u[3];
+    moments<L, T, S::fast>(p, c, f, F, r, u);
+    rr = (float)r;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) uu[d] = (float)u[d];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Moment cache of a particle's box.  The markers of a particle lie within R
+// of its centre cx (R: the ceiling of its largest body-frame marker distance,
+// plus a margin for float rounding), so a stencil node n of a marker x, with
+// floor(x) - 1 <= n <= floor(x) + 2 and floor(cx) - R <= floor(x) <=
+// floor(cx) + R, lies in the cube [o, o + S)^3 with o = floor(cx) - (R + 1)
+// and S = 2R + 4.  One pass (box_node) stores (rho, ux, uy, uz) of every cube
+// node inside the lattice, computed as state_node_moments computes it; the
+// interpolation then reads each node's moments once per marker instead of
+// pulling Q populations per covering marker (about 20 markers per node on a
+// marker spacing of one).  A node outside the cube (a stencil clamped at a
+// face, a centre that is not finite) is pulled as before, so the result is the
+// same bit for bit either way.
+// Box row of a particle, int32[3]: {cache offset in nodes, R (-1: no box),
+// first 256-node chunk of the box in the chunk list}; a box that does not fit
+// the cache's `cap` nodes is not used.
+// ---------------------------------------------------------------------------
+constexpr int kBoxChunk = 256;
+
+struct Box {
+  int o[3];
+  int S;
+  int off;
+  bool ok;
+};
+
+__device__ inline Box box_of(const double* __restrict__ P, const int* __restrict__ row, int cap) {
+  Box b;
+  const int R = row[1];
+  b.off = row[0];
+  b.S = 2 * R + 4;
+  // R <= 512 keeps S^3 in int; the box must lie inside the cache
+  b.ok = R >= 0 && R <= 512 && b.off >= 0 &&
+         (int64_t)b.off + (int64_t)b.S * b.S * b.S <= (int64_t)cap;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const double c = floor(P[FUNLBM_P_CX + d]);
+    const bool fin = c > -1048576.0 && c < 1048576.0;  // false for NaN
+    b.ok = b.ok && fin;
+    b.o[d] = fin ? (int)c - (R + 1) : 0;
+  }
+  return b;
+}
+
+// Cube node `local` of the box of chunk c (one thread per node): moments into
+// cache[off + local].  The chunk list names each chunk's particle.
+template <class L, class S>
+__device__ inline void box_node(const KParams& p, const typename S::elem* __restrict__ state,
+                                const float* __restrict__ fol_old, const float* __restrict__ rho_face,
+                                const double* __restrict__ P, const int* __restrict__ row, int chunk,
+                                float4* __restrict__ cache, int cap) {
+  const Box b = box_of(P, row, cap);
+  if (!b.ok) return;
+  const int local = (chunk - row[2]) * kBoxChunk + (int)threadIdx.x;
+  if (local >= b.S * b.S * b.S) return;
+  const int k = local % b.S, t = local / b.S;
+  const int j = t % b.S, i = t / b.S;
+  const int x = b.o[0] + i, y = b.o[1] + j, z = b.o[2] + k;
+  if (x < 0 || x >= p.nx || y < 0 || y >= p.ny || z < 0 || z >= p.nz) return;
+  float rr, uu[3];
+  state_node_moments<L, S>(p, state, fol_old, rho_face, x, y, z, rr, uu);
+  cache[b.off + local] = make_float4(rr, uu[0], uu[1], uu[2]);
+}
+
+// ---------------------------------------------------------------------------
 // Interpolation + direct forcing (src/funlbm/lbm/lbm3d.py:66-100):
 //   s_u = sum_nodes sum_d u_d w (the three components summed into one scalar),
 //   s_rho = sum w rho, lu = cu + cw x (lx - cx), lF = 2 s_rho (lu - s_u)/dt.
-// FROM_STATE: u', rho' come from pulling the post-stream populations out of
-// the fused state and forming the moments with FOL_old; otherwise from the
-// explicit fields u / rho.  Call with all 64 lanes of the wave.
+// FROM_STATE: u', rho' come from the fused state (state_node_moments, or the
+// box cache where box_row and cache are given and the node is in the box);
+// otherwise from the explicit fields u / rho.  Call with all 64 lanes of the
+// wave.
 // ---------------------------------------------------------------------------
 template <class L, class S, bool FROM_STATE>
 __device__ inline void interp_marker(const KParams& p, int m, int lane,
@@ -109,37 +210,32 @@ __device__ inline void interp_marker(const KParams& p, int m, int lane,
                                      const float* __restrict__ rhofield,
                                      const float* __restrict__ lx, const double* __restrict__ P,
                                      float dt, bool check_sum, float* __restrict__ lu,
-                                     float* __restrict__ lF, int* __restrict__ fail_flag) {
-  using T = typename S::T;
+                                     float* __restrict__ lF, int* __restrict__ fail_flag,
+                                     const int* __restrict__ box_row = nullptr,
+                                     const float4* __restrict__ cache = nullptr, int cap = 0) {
   const float x[3] = {lx[3 * m], lx[3 * m + 1], lx[3 * m + 2]};
   const LaneW lw = lane_weight(p, x, lane);
   float su = 0.0f, sr = 0.0f;
   if (lw.valid) {
-    const int64_t node = lin(p, lw.x, lw.y, lw.z);
     float uu[3], rr;
-    if constexpr (FROM_STATE && S::fast) {
-      // a 16-bit state: moments of the scaled deviations, as its fused step
-      const Node c{lw.x, lw.y, lw.z};
-      float g[L::Q];
-      pull_node<L, DevOf<S>, false>(p, state, c, node, rho_face, g);
-      float F[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) F[d] = fol_old ? fol_old[(int64_t)d * p.n + node] + p.g[d] : p.g[d];
-      moments_dev<L>(p, c, g, F, rr, uu);
-    } else if constexpr (FROM_STATE) {
-      const Node c{lw.x, lw.y, lw.z};
-      T f[L::Q];
-      pull_node<L, S, false>(p, state, c, node, rho_face, f);
-      T F[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-        F[d] = (T)(fol_old ? fol_old[(int64_t)d * p.n + node] + p.g[d] : p.g[d]);
-      T r, u[3];
-      moments<L, T, S::fast>(p, c, f, F, r, u);
-      rr = (float)r;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) uu[d] = (float)u[d];
+    if constexpr (FROM_STATE) {
+      bool hit = false;
+      if (cache) {
+        const Box b = box_of(P, box_row, cap);
+        const int i = lw.x - b.o[0], j = lw.y - b.o[1], k = lw.z - b.o[2];
+        hit = b.ok && (unsigned)i < (unsigned)b.S && (unsigned)j < (unsigned)b.S &&
+              (unsigned)k < (unsigned)b.S;
+        if (hit) {
+          const float4 v = cache[b.off + (i * b.S + j) * b.S + k];
+          rr = v.x;
+          uu[0] = v.y;
+          uu[1] = v.z;
+          uu[2] = v.w;
+        }
+      }
+      if (!hit) state_node_moments<L, S>(p, state, fol_old, rho_face, lw.x, lw.y, lw.z, rr, uu);
     } else {
+      const int64_t node = lin(p, lw.x, lw.y, lw.z);
       rr = rhofield[node];
 #pragma unroll
       for (int d = 0; d < 3; ++d) uu[d] = ufield[(int64_t)d * p.n + node];

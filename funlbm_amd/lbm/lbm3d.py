@@ -1,7 +1,8 @@
 """3-D solver with immersed-boundary particles (src/funlbm/lbm/lbm3d.py:11-264).
 
 run_step (fused, the default) issues per step:
-  [particles]  ibm_interp (pull + moments with FOL_old at the marker stencils),
+  [particles]  ibm_interp_boxed (pull + moments with FOL_old once per node of
+               each particle's box, then the marker stencils read them),
                wall_spring, particle_forces (double), ibm_spread into FOL_new
   fused_step   stream, forcing with FOL_new, moments, collide -- one pass
   [particles]  particle_pose (double rotation, float markers)
@@ -12,12 +13,16 @@ sequence with one kernel per method; both give the same results.
 import torch
 
 from .. import _native as nat
+from ..particle import boxes
 from .base import LBMBase
 
 
 class LBMD3(LBMBase):
     def __init__(self, *args, **kwargs):
         super().__init__(*args, **kwargs)
+        # interpolation from the fused state through the particles' moment
+        # boxes (same results as the per-node pull, which False selects)
+        self.boxed_interp = True
 
     def _init(self):
         """flow.init, one extra collision, particles (src/funlbm/lbm/lbm3d.py:23-30)."""
@@ -34,12 +39,31 @@ class LBMD3(LBMBase):
             ]
             self._origin_ok = [False, False]
             self._tiles = torch.zeros((self.flow.n + 255) // 256, device=dev, dtype=torch.int32)
+            # moment boxes of the interpolation from the fused state
+            radii = [boxes.box_radius(p._lagrange) for p in sw.particles]
+            rows, chunks, nodes = boxes.layout(radii)
+            self._box_rows = torch.from_numpy(rows.reshape(-1)).to(dev)
+            self._box_chunks = torch.from_numpy(chunks).to(dev) if len(chunks) else None
+            self._box_cache = torch.empty(4 * max(nodes, 1), device=dev, dtype=torch.float32)
+            self._box_nodes = nodes
 
     # ------------------------------------------------------------------
     # IBM phases
     # ------------------------------------------------------------------
     def _interp(self, from_state: bool):
         sw, fl = self.particle_swarm, self.flow
+        if from_state and self.boxed_interp:
+            state, rface = fl.state_source()
+            nch = 0 if self._box_chunks is None else int(self._box_chunks.numel())
+            nat.call(
+                "funlbm_ibm_interp_boxed", fl.lat, nat.ptr(state), nat.ptr(fl.fol_old()), nat.ptr(rface),
+                sw.n_markers, nat.ptr(sw.lx), nat.ptr(sw.pid), len(sw.particles), nat.ptr(sw.pstate),
+                nat.ptr(self._box_rows), nch, nat.ptr(self._box_chunks) if nch else None,
+                nat.ptr(self._box_cache), self._box_nodes, float(self.config.dt),
+                1 if self.step > 100 else 0, nat.ptr(sw.lu), nat.ptr(sw.lF), nat.ptr(sw.ctl),
+                fl._stream(),
+            )
+            return
         if from_state:
             state, rface = fl.state_source()
             args = (nat.ptr(state), nat.ptr(fl.fol_old()), nat.ptr(rface), None, None)

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define FUNLBM_ABI_VERSION 2
+#define FUNLBM_ABI_VERSION 3
 
 /* Boundary codes: the values of the reference enum BoundaryCondition
  * (src/funlbm/config/base.py:22-28). */
@@ -229,6 +229,26 @@ int funlbm_ibm_interp(const funlbm_lattice* lat, const void* state, const float*
                       float dt, int32_t check_weight_sum, float* lu, float* lF, int32_t* ctl,
                       void* stream);
 
+/* funlbm_ibm_interp from the fused state, with the moments of each particle's
+ * box computed once per node (the same flow_to_lagrange,
+ * src/funlbm/lbm/lbm3d.py:46-100; the results equal funlbm_ibm_interp's bit
+ * for bit).  particle_box is int32[n_particles][3] = {offset of the box in
+ * box_cache (nodes), R, first chunk}: the box of a particle whose markers lie
+ * within R of its centre is the cube of side 2R + 4 from floor(centre) - R - 1,
+ * (2R + 4)^3 nodes of 16 bytes from box_cache + 4*offset; R = -1 gives no box
+ * (its markers pull per node).  chunk_particle[n_chunks] names the particle of
+ * each 256-node chunk, ceil((2R + 4)^3 / 256) consecutive chunks per box from
+ * its first chunk.  box_cache holds cache_nodes float4 (16-byte aligned); a box
+ * beyond it is not used.  The cache is scratch: its contents after the call
+ * are the boxes' moments of this state. */
+int funlbm_ibm_interp_boxed(const funlbm_lattice* lat, const void* state, const float* fol_old,
+                            const float* rho_face, int32_t n_markers, const float* lx,
+                            const int32_t* marker_particle, int32_t n_particles,
+                            const double* pstate, const int32_t* particle_box, int32_t n_chunks,
+                            const int32_t* chunk_particle, float* box_cache, int32_t cache_nodes,
+                            float dt, int32_t check_weight_sum, float* lu, float* lF, int32_t* ctl,
+                            void* stream);
+
 /* Wall spring, replaces LBMD3.particle_to_wall (src/funlbm/lbm/lbm3d.py:121-167). */
 int funlbm_wall_spring(const funlbm_lattice* lat, int32_t n_particles, const int32_t* marker_offsets,
                        const float* lx, float* lF, float dx, void* stream);
@@ -293,15 +313,18 @@ int funlbm_case_pack(const funlbm_lattice* lat, int64_t f_offset, int64_t node_o
  * single-lattice sequence (interp, wall, forces, spreading, fused step, pose)
  * as persistent kernels over device-side work lists, then the per-case
  * counters advance (parity flips, left -1, done +1) and hdr[4] (the spreading
- * stamp) +1.  hdr is int32[8] device memory: [0] active tiles in
+ * stamp) +1.  hdr is int32[9] device memory: [0] active tiles in
  * tile_case/tile_local, [1] marker slots, [2] particle slots, [3] case slots,
  * [4] stamp, [5], [6], [7] the ends of the D3Q13, D3Q15 and D3Q19 tiles: the
  * tile list holds the tiles of D3Q13 cases first, then D3Q15, D3Q19 and D3Q27
- * ones (one fused kernel per lattice takes its range).  A slot whose case
- * index is -1 is skipped.  particle_markers holds
- * [begin, end) marker ranges (two int32 per particle).  traj (may be NULL)
- * receives float[4] = (cx, cy, cz, step) per particle and step in a ring of
- * traj_cap[0] entries.  The grid sizes fix the persistent launch shapes. */
+ * ones (one fused kernel per lattice takes its range), [8] entries of
+ * chunk_particle.  A slot whose case index is -1 is skipped.  particle_markers
+ * holds [begin, end) marker ranges (two int32 per particle).  traj (may be
+ * NULL) receives float[4] = (cx, cy, cz, step) per particle and step in a ring
+ * of traj_cap[0] entries.  box_cache (may be NULL: every marker pulls per
+ * node), particle_box (int32[3] per particle slot) and chunk_particle are the
+ * moment boxes of funlbm_ibm_interp_boxed, over particle slots.  The grid
+ * sizes fix the persistent launch shapes. */
 int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, int32_t* case_par,
                       int32_t* case_left, int32_t* case_done, int32_t* case_flag,
                       const int32_t* tile_case, const int32_t* tile_local,
@@ -312,7 +335,9 @@ int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, in
                       int32_t* origin_b, int32_t* origin_ok, const float* lag, float* lx, float* lu,
                       float* lF, const float* lm, double* pstate, float* traj,
                       const int32_t* traj_cap, int32_t grid_tiles, int32_t grid_markers,
-                      int32_t grid_particles, int32_t with_ibm, void* stream);
+                      int32_t grid_particles, int32_t with_ibm, const int32_t* particle_box,
+                      const int32_t* chunk_particle, float* box_cache, int32_t cache_nodes,
+                      void* stream);
 
 /* min, mean and max of the reference's f, u and rho after the last fused
  * step, computed from the state without materialising the fields (the

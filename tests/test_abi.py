@@ -36,7 +36,7 @@ def test_bindings_cover_header():
 
 def test_abi_version_and_struct_size():
     lib = nat.lib()
-    assert lib.funlbm_abi_version() == 2
+    assert lib.funlbm_abi_version() == 3
     # 5 ints + 6 ints + 1 int + 3 floats + 3 floats + 18 floats + rounding + seed
     assert ctypes.sizeof(nat.Lattice) == 4 * (5 + 6 + 1 + 3 + 3 + 18 + 2)
 
@@ -161,3 +161,32 @@ def test_deviation_storage_needs_unit_rho0():
     assert nat.lib().funlbm_state_bytes(ctypes.byref(lat), ctypes.byref(nbytes)) == 0
     lat.storage, lat.rho0 = nat.STORE_F32, 0.0
     assert nat.lib().funlbm_state_bytes(ctypes.byref(lat), ctypes.byref(nbytes)) == 0
+
+
+def test_boxed_interp_argument_errors_need_no_device():
+    L = nat.lib()
+    lat = nat.Lattice()
+    lat.q = 19
+    lat.nx = lat.ny = lat.nz = 8
+    lat.tau = 0.8
+    lat.rho0 = 1.0
+    lat.dev_scale = 1024.0
+    for i in range(6):
+        lat.bc[i] = nat.BC_PERIODICAL
+    p = ctypes.c_void_p(4096)
+    lp = ctypes.byref(lat)
+    args = lambda **kw: [kw.get(k, v) for k, v in (  # noqa: E731
+        ("lat", lp), ("state", p), ("fol", p), ("rface", None), ("nm", 10), ("lx", p), ("pid", p),
+        ("np", 1), ("ps", p), ("box", p), ("nch", 4), ("chunks", p), ("cache", p), ("cap", 1024),
+        ("dt", 1.0), ("check", 0), ("lu", p), ("lF", p), ("ctl", p), ("stream", None))]
+    f = L.funlbm_ibm_interp_boxed
+    assert f(*args(nm=0)) == 0  # nothing to do: no launch
+    assert f(*args(nm=-1)) == nat.EINVAL
+    assert f(*args(nch=-1)) == nat.EINVAL
+    assert f(*args(cap=-1)) == nat.EINVAL
+    assert f(*args(state=None)) == nat.EINVAL
+    assert f(*args(box=None)) == nat.EINVAL
+    assert f(*args(chunks=None)) == nat.EINVAL
+    assert f(*args(cache=ctypes.c_void_p(4096 + 4))) == nat.EINVAL
+    assert "aligned" in nat.last_error()
+    assert f(*args(dt=0.0)) == nat.EINVAL

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 import torch
 
+from funlbm_amd import _native as nat
 from oracle import ibm_oracle as ibo
 from oracle import lbm_oracle as orc
 from tests.helpers import flow_json, rel_linf
@@ -197,3 +198,60 @@ def test_async_records_stop_the_run_on_failure():
     lbm.run()
     assert lbm.run_status is False
     assert 101 <= lbm.step <= 101 + 9
+
+
+BOX_PARTS = [
+    # stencils clamped at the x = 0 wall: those lanes fall outside the box
+    {"type": "sphere", "r": 3, "rou": 1.5, "coord": {"center": [2.6, 11, 11]}},
+    {"type": "ellipsoid", "a": 3, "b": 3, "c": 5, "rou": 1.2,
+     "coord": {"center": [13.3, 11.7, 12.2], "alpha": 0.3, "beta": 0.2, "gamma": 0.1}},
+    # overlapping the ellipsoid's stencils: both boxes cover shared nodes
+    {"type": "sphere", "r": 2.5, "rou": 1.1, "coord": {"center": [19.5, 12.4, 10.9]}},
+]
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16", "f64"])
+@pytest.mark.parametrize("q", [19, 27])
+def test_boxed_interpolation_equals_per_node_pull(q, storage):
+    """The interpolation through the particles' moment boxes gives the same
+    bits as pulling every stencil node per marker: marker forces, particle
+    states and populations, over steps whose markers start at the origin
+    (step 1: every lane outside the boxes), then lie in them, with stencils
+    clamped at a wall and boxes that overlap."""
+    size = (27, 23, 23)
+    bcs = (W, W, P, P, P, P)
+    runs = []
+    for boxed in (True, False):
+        lbm = make_case(q=q, size=size, mu=0.05 / 3, bcs=bcs, particles=BOX_PARTS, gl=1e-5,
+                        extra_flow={"storage": storage})
+        lbm.boxed_interp = boxed
+        if boxed:
+            assert lbm._box_chunks is not None and lbm._box_nodes > 0
+            lbm._box_cache.fill_(float("nan"))
+        for s in range(1, 13):
+            lbm.step = s
+            lbm.run_step()
+        torch.cuda.synchronize()
+        runs.append(lbm)
+    a, b = runs
+    sa, sb = a.particle_swarm, b.particle_swarm
+    assert torch.equal(sa.lF, sb.lF)
+    assert torch.equal(sa.lu, sb.lu)
+    assert torch.equal(sa.pstate, sb.pstate)
+    assert torch.equal(a.flow.f, b.flow.f)  # the state's nodes (its slack tiles hold no data)
+    # the last box pass wrote every node of each box inside the lattice (the
+    # cache started as NaN; the centre moved less than a node since then)
+    rows = a._box_rows.view(-1, 3).cpu().numpy()
+    cxs = sa.pstate[:, nat.P_CX : nat.P_CX + 3].cpu().numpy()
+    cache = a._box_cache.view(-1, 4).cpu().numpy()
+    nodes = np.array(size) + 1
+    for (off, R, _), cx in zip(rows, cxs):
+        assert R >= 0
+        S = 2 * R + 4
+        o = np.floor(cx).astype(int) - (R + 1)
+        ii, jj, kk = np.meshgrid(*(np.arange(S),) * 3, indexing="ij")
+        inside = np.ones_like(ii, dtype=bool)
+        for d, ax in enumerate((ii, jj, kk)):
+            inside &= (o[d] + ax >= 0) & (o[d] + ax < nodes[d])
+        vals = cache[off : off + S ** 3].reshape(S, S, S, 4)
+        assert np.isfinite(vals[inside]).all()
