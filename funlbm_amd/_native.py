@@ -100,8 +100,8 @@ SIGNATURES = {
     ),
     "funlbm_ibm_interp_boxed": (
         c_int32,
-        [LATP, P, P, P, c_int32, P, P, c_int32, P, P, c_int32, P, P, c_int32, c_float, c_int32, P,
-         P, P, P],
+        [LATP, P, P, P, c_int32, P, P, c_int32, P, P, c_int32, P, P, c_int32, P, c_float, c_int32,
+         P, P, P, P],
     ),
     "funlbm_wall_spring": (c_int32, [LATP, c_int32, P, P, P, c_float, P]),
     "funlbm_particle_forces": (c_int32, [c_int32, P, P, P, P, P, ctypes.c_double, ctypes.c_double, P]),
@@ -111,7 +111,7 @@ SIGNATURES = {
     "funlbm_flow_stats": (c_int32, [LATP, P, P, P, P, P, P, P, c_int32, P, P]),
     "funlbm_case_bytes": (c_int32, []),
     "funlbm_case_pack": (c_int32, [LATP, c_int64, c_int64, c_int32, P]),
-    "funlbm_batch_step": (c_int32, [c_int32] + [P] * 30 + [c_int32] * 4 + [P, P, P, c_int32, P]),
+    "funlbm_batch_step": (c_int32, [c_int32] + [P] * 30 + [c_int32] * 4 + [P, P, P, c_int32, P, P]),
 }
 
 _lib = None

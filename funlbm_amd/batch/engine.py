@@ -203,6 +203,7 @@ class BatchEngine:
         self.particle_box = z(3 * int(max_particles), i32)
         self.chunk_particle = torch.full((self.box_cap // boxes.CHUNK + int(max_particles),), -1,
                                          device=dev, dtype=i32)
+        self.box_miss = z(int(max_markers) + 1, i32)
         self.traj = z(int(max_particles) * self.traj_cap * 4, f32)
         self.traj_cap_t = torch.tensor([self.traj_cap], device=dev, dtype=i32)
         self.slots = {}
@@ -380,7 +381,7 @@ class BatchEngine:
             P(self.origin[1]), P(self.origin_ok), P(self.lag), P(self.lx), P(self.lu), P(self.lF),
             P(self.lm), P(self.pstate), P(self.traj), P(self.traj_cap_t), int(g["tiles"]),
             int(g["markers"]), int(g["particles"]), 1, P(self.particle_box), P(self.chunk_particle),
-            P(self.box_cache), self.box_cap, nat.stream_handle(self.device),
+            P(self.box_cache), self.box_cap, P(self.box_miss), nat.stream_handle(self.device),
         )
 
     def step(self, n: int = 1):
@@ -400,7 +401,7 @@ class BatchEngine:
         cus = torch.cuda.get_device_properties(dev).multi_processor_count
         n_tiles = int(self.hdr[H_TILES].item())
         n_mark = max(1, int(self.hdr[H_MARKERS].item()))
-        self.grid["markers"] = max(1, min(-(-n_mark // 4), 8 * cus))
+        self.grid["markers"] = max(1, min(-(-n_mark // 16), 8 * cus))
         self.grid["particles"] = max(1, min(int(self.hdr[H_PARTICLES].item()), 4 * cus))
         if candidates is None:
             candidates = sorted({c for c in (2 * cus, 4 * cus, 8 * cus, 16 * cus, n_tiles) if 0 < c <= max(n_tiles, 1)})

@@ -107,20 +107,21 @@ __global__ __launch_bounds__(kBlock) void k_batch_fused(
 // ---------------------------------------------------------------------------
 // IBM over all markers / particles of stepping cases.
 // ---------------------------------------------------------------------------
+// Per-node interpolation from the state: of every marker (list == NULL) or
+// of the markers in list ([0] = count, then marker indices).
 template <class S>
-__global__ __launch_bounds__(kIbmBlock) void k_batch_interp(
+__global__ __launch_bounds__(kIbmBlock) void k_batch_interp_pull(
     const CaseDesc* __restrict__ cases, const int* __restrict__ hdr, BatchDyn dyn,
     const int* __restrict__ marker_case, const int* __restrict__ marker_particle,
     const typename S::elem* __restrict__ pool_a, const typename S::elem* __restrict__ pool_b,
     const float* __restrict__ fol_a, const float* __restrict__ fol_b,
     const float* __restrict__ lx, const double* __restrict__ ps, float* __restrict__ lu,
-    float* __restrict__ lF, const int* __restrict__ pbox, const float4* __restrict__ cache, int cap) {
-  const int nm = hdr[H_MARKERS];
-  const int lane = threadIdx.x & (kWave - 1);
-  const int waves = gridDim.x * (kIbmBlock / kWave);
-  // consecutive markers (one particle's, sharing box nodes) on one XCD's L2
-  const int b = (int)xcd_block(blockIdx.x, gridDim.x);
-  for (int m = b * (kIbmBlock / kWave) + threadIdx.x / kWave; m < nm; m += waves) {
+    float* __restrict__ lF, const int* __restrict__ list) {
+  const int n = list ? list[0] : hdr[H_MARKERS];
+  const int sub = threadIdx.x & (kMarkerLanes - 1);
+  const int stride = gridDim.x * kMarkersPerBlock;
+  for (int e = blockIdx.x * kMarkersPerBlock + (int)threadIdx.x / kMarkerLanes; e < n; e += stride) {
+    const int m = list ? list[1 + e] : e;
     const int c = marker_case[m];
     if (c < 0 || dyn.left[c] <= 0) continue;
     const CaseDesc& cd = cases[c];
@@ -130,23 +131,45 @@ __global__ __launch_bounds__(kIbmBlock) void k_batch_interp(
     const bool check = dyn.done[c] + 1 > 100;
     with_lattice(cd.q, [&](auto l) {
       using L = decltype(l);
-      const int pi = marker_particle[m];
-      interp_marker<L, S, true>(cd.p, m, lane, st, fo, nullptr, nullptr, nullptr, lx, pst(ps, pi),
-                                1.0f, check, lu, lF, &dyn.flag[c], cache ? pbox + 3 * pi : nullptr,
-                                cache, cap);
+      interp_marker<L, S, true>(cd.p, m, sub, st, fo, nullptr, nullptr, nullptr, lx,
+                                pst(ps, marker_particle[m]), 1.0f, check, lu, lF, &dyn.flag[c]);
     });
   }
 }
 
+// Interpolation from the particles' moment boxes; markers with a node outside
+// their box go to miss_list for k_batch_interp_pull.
+__global__ __launch_bounds__(kIbmBlock) void k_batch_interp_boxed(
+    const CaseDesc* __restrict__ cases, const int* __restrict__ hdr, BatchDyn dyn,
+    const int* __restrict__ marker_case, const int* __restrict__ marker_particle,
+    const float* __restrict__ lx, const double* __restrict__ ps, float* __restrict__ lu,
+    float* __restrict__ lF, const int* __restrict__ pbox, const float4* __restrict__ cache, int cap,
+    int* __restrict__ miss_list) {
+  const int nm = hdr[H_MARKERS];
+  const int sub = threadIdx.x & (kMarkerLanes - 1);
+  const int stride = gridDim.x * kMarkersPerBlock;
+  // consecutive markers (one particle's, sharing box nodes) on one XCD's L2
+  const int b = (int)xcd_block(blockIdx.x, gridDim.x);
+  for (int m = b * kMarkersPerBlock + (int)threadIdx.x / kMarkerLanes; m < nm; m += stride) {
+    const int c = marker_case[m];
+    if (c < 0 || dyn.left[c] <= 0) continue;
+    const int pi = marker_particle[m];
+    interp_marker_boxed(cases[c].p, m, sub, lx, pst(ps, pi), 1.0f, dyn.done[c] + 1 > 100, lu, lF,
+                        &dyn.flag[c], pbox + 3 * pi, cache, cap, miss_list);
+  }
+}
+
 // Moment boxes of the active cases' particles (box_node), before
-// k_batch_interp reads them.  chunk_particle names each chunk's particle.
+// k_batch_interp_boxed reads them.  chunk_particle names each chunk's particle.
 template <class S>
 __global__ __launch_bounds__(kBoxChunk) void k_batch_box_moments(
     const CaseDesc* __restrict__ cases, const int* __restrict__ hdr, BatchDyn dyn,
     const int* __restrict__ particle_case, const typename S::elem* __restrict__ pool_a,
     const typename S::elem* __restrict__ pool_b, const float* __restrict__ fol_a,
     const float* __restrict__ fol_b, const double* __restrict__ ps, const int* __restrict__ pbox,
-    const int* __restrict__ chunk_particle, float4* __restrict__ cache, int cap) {
+    const int* __restrict__ chunk_particle, float4* __restrict__ cache, int cap,
+    int* __restrict__ miss_list) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) miss_list[0] = 0;  // k_batch_interp_boxed appends
   const int nch = hdr[H_CHUNKS];
   const int np = hdr[H_PARTICLES];
   for (int ch = blockIdx.x; ch < nch; ch += gridDim.x) {
@@ -201,22 +224,37 @@ __global__ __launch_bounds__(kIbmBlock) void k_batch_spread_prep(
     const float* __restrict__ lx, const int* __restrict__ origin_a, const int* __restrict__ origin_b,
     const int* __restrict__ origin_ok, float* __restrict__ fol_a, float* __restrict__ fol_b,
     unsigned long long* __restrict__ owner) {
+  __shared__ PrepPeer peers[kIbmBlock / kWave][kMarkersPerWave];
   const int nm = hdr[H_MARKERS];
   const uint32_t stamp = (uint32_t)hdr[H_STAMP] + 1u;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int waves = gridDim.x * (kIbmBlock / kWave);
-  for (int m = blockIdx.x * (kIbmBlock / kWave) + threadIdx.x / kWave; m < nm; m += waves) {
-    const int c = marker_case[m];
-    if (c < 0 || dyn.left[c] <= 0) continue;
-    const CaseDesc& cd = cases[c];
-    const int par = dyn.par[c];
-    // FOL_new of this step is buffer 1 - par; its origins were recorded the
-    // last time the case wrote that buffer
-    float* fn = (par ? fol_a : fol_b) + 3 * cd.node_off;
-    const int* org = par ? origin_a : origin_b;
-    const bool ok = (origin_ok[c] >> (1 - par)) & 1;
-    spread_prep_marker(cd.p, m, m - case_marker_begin[c], lane, lx, ok ? org : nullptr, fn,
-                       owner + cd.node_off, stamp);
+  const int sub = threadIdx.x & (kMarkerLanes - 1);
+  const int wv = threadIdx.x / kWave;
+  const int g = (threadIdx.x & (kWave - 1)) / kMarkerLanes;
+  const int stride = gridDim.x * kMarkersPerBlock;
+  // the loop bound is uniform over the wave: its four markers take part in
+  // every pass together (spread_prep_marker shares their stencils)
+  for (int mb = blockIdx.x * kMarkersPerBlock + wv * kMarkersPerWave; mb < nm; mb += stride) {
+    const int m = mb + g;
+    int c = -1;
+    bool act = m < nm;
+    if (act) {
+      c = marker_case[m];
+      act = c >= 0 && dyn.left[c] > 0;
+    }
+    const CaseDesc& cd = cases[act ? c : 0];
+    float* fn = nullptr;
+    const int* org = nullptr;
+    int key = 0;
+    if (act) {
+      const int par = dyn.par[c];
+      // FOL_new of this step is buffer 1 - par; its origins were recorded the
+      // last time the case wrote that buffer
+      fn = (par ? fol_a : fol_b) + 3 * cd.node_off;
+      const bool ok = (origin_ok[c] >> (1 - par)) & 1;
+      org = ok ? (par ? origin_a : origin_b) : nullptr;
+      key = m - case_marker_begin[c];
+    }
+    spread_prep_marker(cd.p, act, c, m, key, sub, lx, org, fn, owner + cd.node_off, stamp, peers[wv]);
   }
 }
 
@@ -229,18 +267,18 @@ __global__ __launch_bounds__(kIbmBlock) void k_batch_spread_write(
     const unsigned long long* __restrict__ owner, uint32_t* __restrict__ fol_tiles) {
   const int nm = hdr[H_MARKERS];
   const uint32_t stamp = (uint32_t)hdr[H_STAMP] + 1u;
-  const int lane = threadIdx.x & (kWave - 1);
-  const int waves = gridDim.x * (kIbmBlock / kWave);
-  for (int m = blockIdx.x * (kIbmBlock / kWave) + threadIdx.x / kWave; m < nm; m += waves) {
+  const int sub = threadIdx.x & (kMarkerLanes - 1);
+  const int stride = gridDim.x * kMarkersPerBlock;
+  for (int m = blockIdx.x * kMarkersPerBlock + (int)threadIdx.x / kMarkerLanes; m < nm; m += stride) {
     const int c = marker_case[m];
     if (c < 0 || dyn.left[c] <= 0) continue;
     const CaseDesc& cd = cases[c];
     const int par = dyn.par[c];
     float* fn = (par ? fol_a : fol_b) + 3 * cd.node_off;
     int* org = par ? origin_a : origin_b;
-    spread_write_marker(cd.p, m, m - case_marker_begin[c], lane, lx, lF, lm, fn, owner + cd.node_off,
+    spread_write_marker(cd.p, m, m - case_marker_begin[c], sub, lx, lF, lm, fn, owner + cd.node_off,
                         org, fol_tiles + cd.tile_begin, stamp);
-    if (lane == 0 && m == case_marker_begin[c]) atomicOr(&origin_ok[c], 1 << (1 - par));
+    if (sub == 0 && m == case_marker_begin[c]) atomicOr(&origin_ok[c], 1 << (1 - par));
   }
 }
 
@@ -326,7 +364,7 @@ int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, in
                       const int32_t* traj_cap, int32_t grid_tiles, int32_t grid_markers,
                       int32_t grid_particles, int32_t with_ibm, const int32_t* particle_box,
                       const int32_t* chunk_particle, float* box_cache, int32_t cache_nodes,
-                      void* stream) {
+                      int32_t* miss_list, void* stream) {
   if (!cases || !hdr || !case_par || !case_left || !case_done || !case_flag || !tile_case ||
       !tile_local || !pool_a || !pool_b || !fol_a || !fol_b || !fol_tiles)
     return set_error(FUNLBM_EINVAL, "batch_step: NULL array");
@@ -336,7 +374,7 @@ int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, in
                    !particle_markers || !owner || !origin_a || !origin_b || !origin_ok || !lag ||
                    !lx || !lu || !lF || !lm || !pstate))
     return set_error(FUNLBM_EINVAL, "batch_step: NULL particle array");
-  if (with_ibm && box_cache && (!particle_box || !chunk_particle || cache_nodes < 0))
+  if (with_ibm && box_cache && (!particle_box || !chunk_particle || !miss_list || cache_nodes < 0))
     return set_error(FUNLBM_EINVAL, "batch_step: box cache without its tables");
   if (box_cache && ((uintptr_t)box_cache & 15))
     return set_error(FUNLBM_EINVAL, "batch_step: box cache must be 16-byte aligned");
@@ -353,13 +391,21 @@ int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, in
       if (cache) {
         hipLaunchKernelGGL((k_batch_box_moments<S>), dim3(grid_markers), dim3(kBoxChunk), 0, s, cd,
                            hdr, dyn, particle_case, (const E*)pool_a, (const E*)pool_b, fol_a, fol_b,
-                           pstate, particle_box, chunk_particle, (float4*)box_cache, cache_nodes);
+                           pstate, particle_box, chunk_particle, (float4*)box_cache, cache_nodes,
+                           miss_list);
         if ((rc = check_launch("batch_box_moments"))) return rc;
+        hipLaunchKernelGGL(k_batch_interp_boxed, dim3(grid_markers), dim3(kIbmBlock), 0, s, cd, hdr,
+                           dyn, marker_case, marker_particle, lx, pstate, lu, lF, particle_box, cache,
+                           cache_nodes, miss_list);
+        if ((rc = check_launch("batch_interp_boxed"))) return rc;
       }
-      hipLaunchKernelGGL((k_batch_interp<S>), dim3(grid_markers), dim3(kIbmBlock), 0, s, cd, hdr, dyn,
-                         marker_case, marker_particle, (const E*)pool_a, (const E*)pool_b, fol_a,
-                         fol_b, lx, pstate, lu, lF, particle_box, cache, cache_nodes);
-      if ((rc = check_launch("batch_interp"))) return rc;
+      // the markers without a box (all of them without a cache), per node;
+      // the count of the list is on the device: a small fixed grid
+      hipLaunchKernelGGL((k_batch_interp_pull<S>), dim3(cache ? 64 : grid_markers), dim3(kIbmBlock), 0,
+                         s, cd, hdr, dyn, marker_case, marker_particle, (const E*)pool_a,
+                         (const E*)pool_b, fol_a, fol_b, lx, pstate, lu, lF,
+                         cache ? (const int*)miss_list : nullptr);
+      if ((rc = check_launch("batch_interp_pull"))) return rc;
       hipLaunchKernelGGL(k_batch_wall, dim3(grid_particles), dim3(kIbmBlock), 0, s, cd, hdr, dyn,
                          particle_case, particle_markers, lx, lF, 1.0f);
       if ((rc = check_launch("batch_wall"))) return rc;

@@ -5,7 +5,7 @@
 // ibm_body.h).
 //
 // Markers of all particles are concatenated (particle-major, the reference's
-// loop order).  One 64-lane wavefront serves one marker.
+// loop order).  16 lanes serve one marker, four markers per wavefront.
 #include "capi_util.h"
 #include "ibm_body.h"
 #include "lbm_common.h"
@@ -18,17 +18,44 @@ __global__ __launch_bounds__(kIbmBlock) void k_interp(
     const float* __restrict__ rho_face, const float* __restrict__ ufield,
     const float* __restrict__ rhofield, int nm, const float* __restrict__ lx,
     const int* __restrict__ pid, const double* __restrict__ ps, float dt, int check_sum,
-    float* __restrict__ lu, float* __restrict__ lF, int* __restrict__ ctl,
-    const int* __restrict__ pbox, const float4* __restrict__ cache, int cap) {
+    float* __restrict__ lu, float* __restrict__ lF, int* __restrict__ ctl) {
+  const int m = blockIdx.x * kMarkersPerBlock + (int)threadIdx.x / kMarkerLanes;
+  const int sub = threadIdx.x & (kMarkerLanes - 1);
+  if (m >= nm) return;
+  interp_marker<L, S, FROM_STATE>(p, m, sub, state, fol_old, rho_face, ufield, rhofield, lx,
+                                  pst(ps, pid[m]), dt, check_sum != 0, lu, lF, &ctl[1]);
+}
+
+__global__ __launch_bounds__(kIbmBlock) void k_interp_boxed(
+    KParams p, int nm, const float* __restrict__ lx, const int* __restrict__ pid,
+    const double* __restrict__ ps, float dt, int check_sum, float* __restrict__ lu,
+    float* __restrict__ lF, int* __restrict__ ctl, const int* __restrict__ pbox,
+    const float4* __restrict__ cache, int cap, int* __restrict__ miss_list) {
   // consecutive markers (one particle's, sharing box nodes) on one XCD's L2
   const int b = (int)xcd_block(blockIdx.x, gridDim.x);
-  const int m = b * (kIbmBlock / kWave) + threadIdx.x / kWave;
-  const int lane = threadIdx.x & (kWave - 1);
+  const int m = b * kMarkersPerBlock + (int)threadIdx.x / kMarkerLanes;
+  const int sub = threadIdx.x & (kMarkerLanes - 1);
   if (m >= nm) return;
   const int pi = pid[m];
-  interp_marker<L, S, FROM_STATE>(p, m, lane, state, fol_old, rho_face, ufield, rhofield, lx,
-                                  pst(ps, pi), dt, check_sum != 0, lu, lF, &ctl[1],
-                                  cache ? pbox + 3 * pi : nullptr, cache, cap);
+  interp_marker_boxed(p, m, sub, lx, pst(ps, pi), dt, check_sum != 0, lu, lF, &ctl[1], pbox + 3 * pi,
+                      cache, cap, miss_list);
+}
+
+// The markers k_interp_boxed left: per-node interpolation from the state.
+template <class L, class S>
+__global__ __launch_bounds__(kIbmBlock) void k_interp_missed(
+    KParams p, const typename S::elem* __restrict__ state, const float* __restrict__ fol_old,
+    const float* __restrict__ rho_face, const int* __restrict__ miss_list,
+    const float* __restrict__ lx, const int* __restrict__ pid, const double* __restrict__ ps,
+    float dt, int check_sum, float* __restrict__ lu, float* __restrict__ lF, int* __restrict__ ctl) {
+  const int n = miss_list[0];
+  const int sub = threadIdx.x & (kMarkerLanes - 1);
+  for (int e = blockIdx.x * kMarkersPerBlock + (int)threadIdx.x / kMarkerLanes; e < n;
+       e += gridDim.x * kMarkersPerBlock) {
+    const int m = miss_list[1 + e];
+    interp_marker<L, S, true>(p, m, sub, state, fol_old, rho_face, nullptr, nullptr, lx,
+                              pst(ps, pid[m]), dt, check_sum != 0, lu, lF, &ctl[1]);
+  }
 }
 
 // Moment cache of the particles' boxes (box_node), one 256-node chunk per
@@ -38,7 +65,8 @@ __global__ __launch_bounds__(kBoxChunk) void k_box_moments(
     KParams p, const typename S::elem* __restrict__ state, const float* __restrict__ fol_old,
     const float* __restrict__ rho_face, const double* __restrict__ ps, const int* __restrict__ pbox,
     const int* __restrict__ chunk_particle, int n_chunks, int np, float4* __restrict__ cache,
-    int cap) {
+    int cap, int* __restrict__ miss_list) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) miss_list[0] = 0;  // k_interp_boxed appends
   for (int c = blockIdx.x; c < n_chunks; c += gridDim.x) {
     const int pi = chunk_particle[c];
     if ((unsigned)pi >= (unsigned)np) continue;
@@ -76,10 +104,12 @@ __global__ __launch_bounds__(kIbmBlock) void k_spread_prep(KParams p, int nm,
                                                            float* __restrict__ fol,
                                                            unsigned long long* __restrict__ owner,
                                                            const int* __restrict__ ctl) {
-  const int m = blockIdx.x * (kIbmBlock / kWave) + threadIdx.x / kWave;
-  const int lane = threadIdx.x & (kWave - 1);
-  if (m >= nm) return;
-  spread_prep_marker(p, m, m, lane, lx, clear_origin, fol, owner, (uint32_t)(ctl[0] + 1));
+  __shared__ PrepPeer peers[kIbmBlock / kWave][kMarkersPerWave];
+  const int m = blockIdx.x * kMarkersPerBlock + (int)threadIdx.x / kMarkerLanes;
+  const int sub = threadIdx.x & (kMarkerLanes - 1);
+  // every lane takes part (spread_prep_marker shares the wave's stencils)
+  spread_prep_marker(p, m < nm, 0, m, m, sub, lx, clear_origin, fol, owner, (uint32_t)(ctl[0] + 1),
+                     peers[threadIdx.x / kWave]);
 }
 
 __global__ __launch_bounds__(kIbmBlock) void k_spread_write(
@@ -87,15 +117,15 @@ __global__ __launch_bounds__(kIbmBlock) void k_spread_write(
     const float* __restrict__ lm, float* __restrict__ fol,
     const unsigned long long* __restrict__ owner, int* __restrict__ origin,
     uint32_t* __restrict__ tiles, const int* __restrict__ ctl) {
-  const int m = blockIdx.x * (kIbmBlock / kWave) + threadIdx.x / kWave;
-  const int lane = threadIdx.x & (kWave - 1);
+  const int m = blockIdx.x * kMarkersPerBlock + (int)threadIdx.x / kMarkerLanes;
+  const int sub = threadIdx.x & (kMarkerLanes - 1);
   if (m >= nm) return;
-  spread_write_marker(p, m, m, lane, lx, lF, lm, fol, owner, origin, tiles, (uint32_t)(ctl[0] + 1));
+  spread_write_marker(p, m, m, sub, lx, lF, lm, fol, owner, origin, tiles, (uint32_t)(ctl[0] + 1));
 }
 
 __global__ void k_bump(int* ctl) { ctl[0] += 1; }
 
-static int wave_grid(int nm) { return (nm + (kIbmBlock / kWave) - 1) / (kIbmBlock / kWave); }
+static int wave_grid(int nm) { return (nm + kMarkersPerBlock - 1) / kMarkersPerBlock; }
 
 }  // namespace funlbm
 
@@ -131,12 +161,11 @@ int funlbm_ibm_interp(const funlbm_lattice* lat, const void* state, const float*
       if (from_state)
         hipLaunchKernelGGL((k_interp<L, S, true>), dim3(wave_grid(n_markers)), dim3(kIbmBlock), 0, s, p,
                            (const E*)state, fol_old, rho_face, u, rho, n_markers, lx, marker_particle,
-                           pstate, dt, check_weight_sum, lu, lF, ctl, nullptr, nullptr, 0);
+                           pstate, dt, check_weight_sum, lu, lF, ctl);
       else
         hipLaunchKernelGGL((k_interp<L, S, false>), dim3(wave_grid(n_markers)), dim3(kIbmBlock), 0, s,
                            p, (const E*)state, fol_old, rho_face, u, rho, n_markers, lx,
-                           marker_particle, pstate, dt, check_weight_sum, lu, lF, ctl, nullptr,
-                           nullptr, 0);
+                           marker_particle, pstate, dt, check_weight_sum, lu, lF, ctl);
       return check_launch("ibm_interp");
     });
   });
@@ -147,8 +176,8 @@ int funlbm_ibm_interp_boxed(const funlbm_lattice* lat, const void* state, const 
                             const int32_t* marker_particle, int32_t n_particles,
                             const double* pstate, const int32_t* particle_box, int32_t n_chunks,
                             const int32_t* chunk_particle, float* box_cache, int32_t cache_nodes,
-                            float dt, int32_t check_weight_sum, float* lu, float* lF, int32_t* ctl,
-                            void* stream) {
+                            int32_t* miss_list, float dt, int32_t check_weight_sum, float* lu,
+                            float* lF, int32_t* ctl, void* stream) {
   KParams p;
   int rc = make_params(lat, &p);
   if (rc) return rc;
@@ -157,7 +186,7 @@ int funlbm_ibm_interp_boxed(const funlbm_lattice* lat, const void* state, const 
   if (n_markers == 0) return 0;
   if ((rc = reject_neq(p, "ibm_interp from a state"))) return rc;
   if (!state || !lx || !marker_particle || !pstate || !particle_box || !lu || !lF || !ctl ||
-      (n_chunks > 0 && (!chunk_particle || !box_cache)))
+      (n_chunks > 0 && (!chunk_particle || !box_cache || !miss_list)))
     return set_error(FUNLBM_EINVAL, "ibm_interp_boxed: NULL array");
   if (box_cache && ((uintptr_t)box_cache & 15))
     return set_error(FUNLBM_EINVAL, "ibm_interp_boxed: box cache must be 16-byte aligned");
@@ -165,24 +194,31 @@ int funlbm_ibm_interp_boxed(const funlbm_lattice* lat, const void* state, const 
     return set_error(FUNLBM_EINVAL, "ibm_interp: moving wall needs the face density");
   if (!(dt > 0.0f)) return set_error(FUNLBM_EINVAL, "ibm_interp: dt must be positive");
   hipStream_t s = (hipStream_t)stream;
-  const float4* cache = n_chunks > 0 ? (const float4*)box_cache : nullptr;
   return dispatch_lattice(lat->q, [&](auto l) {
     using L = decltype(l);
     return dispatch_storage(lat->storage, [&](auto st) {
       using S = decltype(st);
       using E = typename S::elem;
-      if (cache) {
-        hipLaunchKernelGGL((k_box_moments<L, S>), dim3(n_chunks < 65536 ? n_chunks : 65536),
-                           dim3(kBoxChunk), 0, s, p, (const E*)state, fol_old, rho_face, pstate,
-                           particle_box, chunk_particle, n_chunks, n_particles, (float4*)box_cache,
-                           cache_nodes);
-        if ((rc = check_launch("ibm_box_moments"))) return rc;
+      if (n_chunks == 0) {  // no box: every marker per node
+        hipLaunchKernelGGL((k_interp<L, S, true>), dim3(wave_grid(n_markers)), dim3(kIbmBlock), 0, s,
+                           p, (const E*)state, fol_old, rho_face, nullptr, nullptr, n_markers, lx,
+                           marker_particle, pstate, dt, check_weight_sum, lu, lF, ctl);
+        return check_launch("ibm_interp");
       }
-      hipLaunchKernelGGL((k_interp<L, S, true>), dim3(wave_grid(n_markers)), dim3(kIbmBlock), 0, s, p,
-                         (const E*)state, fol_old, rho_face, nullptr, nullptr, n_markers, lx,
-                         marker_particle, pstate, dt, check_weight_sum, lu, lF, ctl, particle_box,
-                         cache, cache_nodes);
-      return check_launch("ibm_interp");
+      hipLaunchKernelGGL((k_box_moments<L, S>), dim3(n_chunks < 65536 ? n_chunks : 65536),
+                         dim3(kBoxChunk), 0, s, p, (const E*)state, fol_old, rho_face, pstate,
+                         particle_box, chunk_particle, n_chunks, n_particles, (float4*)box_cache,
+                         cache_nodes, miss_list);
+      if ((rc = check_launch("ibm_box_moments"))) return rc;
+      hipLaunchKernelGGL(k_interp_boxed, dim3(wave_grid(n_markers)), dim3(kIbmBlock), 0, s, p,
+                         n_markers, lx, marker_particle, pstate, dt, check_weight_sum, lu, lF, ctl,
+                         particle_box, (const float4*)box_cache, cache_nodes, miss_list);
+      if ((rc = check_launch("ibm_interp_boxed"))) return rc;
+      // the markers left over (their count is on the device): a small fixed grid
+      hipLaunchKernelGGL((k_interp_missed<L, S>), dim3(64), dim3(kIbmBlock), 0, s, p, (const E*)state,
+                         fol_old, rho_face, (const int*)miss_list, lx, marker_particle, pstate, dt,
+                         check_weight_sum, lu, lF, ctl);
+      return check_launch("ibm_interp_missed");
     });
   });
 }

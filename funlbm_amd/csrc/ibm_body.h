@@ -1,7 +1,9 @@
 // Immersed-boundary bodies shared by the single-lattice kernels (ibm.hip)
 // and the multi-lattice kernels (batch.hip).
 //
-// Per marker (one 64-lane wavefront, lane (i, j, k) = stencil node rl + (i, j, k)):
+// Per marker (16 lanes, four markers per wavefront; lane (j, k) of a marker
+// holds the stencil nodes rl + (i, j, k), i = 0..3, so that four consecutive
+// lanes touch four consecutive nodes):
 //   interp_marker       interpolation + direct forcing
 //   spread_prep_marker  clear the old stencil, claim the owner of each node
 //   spread_write_marker the owner writes w lF lm
@@ -16,7 +18,10 @@
 namespace funlbm {
 
 constexpr int kWave = 64;
-constexpr int kIbmBlock = 256;  // 4 markers per block
+constexpr int kIbmBlock = 256;
+constexpr int kMarkerLanes = 16;                        // lanes of one marker
+constexpr int kMarkersPerWave = kWave / kMarkerLanes;   // 4
+constexpr int kMarkersPerBlock = kIbmBlock / kMarkerLanes;  // 16
 constexpr float kPiF = 3.14159265358979323846f;
 
 // Stencil of a marker (src/funlbm/lbm/lbm3d.py:32-44): rl = max(floor(x) - 1, 0),
@@ -55,34 +60,64 @@ __device__ inline float unnormalised_weight(const float x[3], int nx_, int ny_, 
   return (t[0] * t[1]) * t[2];
 }
 
+// Sum over the 16 lanes of one marker (xor butterfly 8, 4, 2, 1 on lane
+// 4j + k: j's bits first, then k's).
 template <class T>
-__device__ inline T wave_sum(T v) {
+__device__ inline T group_sum(T v) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, kMarkerLanes);
   return v;
 }
 
-// Lane geometry of a marker: node coordinates and normalised weight.
+// Sum over a marker's 64 stencil nodes, v[i] being node (i, j, k) of the
+// lane.  The tree is the xor butterfly 32, 16, 8, 4, 2, 1 over node index
+// 16i + 4j + k: its first two levels give (v0 + v2) + (v1 + v3) in each lane,
+// the other four are group_sum.
+template <class T>
+__device__ inline T stencil_sum(const T v[4]) {
+  return group_sum((v[0] + v[2]) + (v[1] + v[3]));
+}
+
+// Lane geometry of a marker: node (x0 + i, y, z) of each layer i, which of
+// them lie in the stencil, and their normalised weights.
 struct LaneW {
-  bool valid;
-  int x, y, z;
-  float w;     // normalised weight (0 for invalid lanes)
-  float wsum;  // sum of normalised weights of the stencil
+  int x0, y, z;
+  unsigned valid;  // bit i: node i inside the stencil
+  float w[4];      // normalised weights (0 outside)
+  float wsum;      // sum of the normalised weights of the stencil
 };
 
-__device__ inline LaneW lane_weight(const KParams& p, const float x[3], int lane) {
+__device__ inline LaneW lane_weight(const KParams& p, const float x[3], int sub) {
   LaneW lw;
   const Stencil s = stencil_of(p, x);
-  const int i = lane >> 4, j = (lane >> 2) & 3, k = lane & 3;
-  lw.x = s.lo[0] + i;
+  const int j = sub >> 2, k = sub & 3;
+  lw.x0 = s.lo[0];
   lw.y = s.lo[1] + j;
   lw.z = s.lo[2] + k;
-  lw.valid = lw.x < s.hi[0] && lw.y < s.hi[1] && lw.z < s.hi[2];
-  float w = lw.valid ? unnormalised_weight(x, lw.x, lw.y, lw.z) : 0.0f;
-  const float ws = wave_sum(w);
-  if (ws > 0.0f) w = w / ws;
-  lw.w = w;
-  lw.wsum = wave_sum(w);
+  const bool vyz = lw.y < s.hi[1] && lw.z < s.hi[2];
+  // unnormalised_weight per node: (t_x t_y) t_z, t_y and t_z shared by the layers
+  const float ry = (float)lw.y - x[1], rz = (float)lw.z - x[2];
+  const bool inyz = fabsf(ry) <= 2.0f && fabsf(rz) <= 2.0f;
+  const float ty = (1.0f + cosf(fabsf(ry * kPiF / 2.0f))) / 4.0f;
+  const float tz = (1.0f + cosf(fabsf(rz * kPiF / 2.0f))) / 4.0f;
+  lw.valid = 0;
+  float w[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const bool v = vyz && lw.x0 + i < s.hi[0];
+    lw.valid |= (unsigned)v << i;
+    const float rx = (float)(lw.x0 + i) - x[0];
+    const float tx = (1.0f + cosf(fabsf(rx * kPiF / 2.0f))) / 4.0f;
+    w[i] = (v && inyz && fabsf(rx) <= 2.0f) ? (tx * ty) * tz : 0.0f;
+  }
+  const float ws = stencil_sum(w);
+  if (ws > 0.0f) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w[i] = w[i] / ws;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) lw.w[i] = w[i];
+  lw.wsum = stencil_sum(w);
   return lw;
 }
 
@@ -138,9 +173,10 @@ __device__ __forceinline__ void state_node_moments(const KParams& p,
 // node inside the lattice, computed as state_node_moments computes it; the
 // interpolation then reads each node's moments once per marker instead of
 // pulling Q populations per covering marker (about 20 markers per node on a
-// marker spacing of one).  A node outside the cube (a stencil clamped at a
-// face, a centre that is not finite) is pulled as before, so the result is the
-// same bit for bit either way.
+// marker spacing of one).  A marker with a node outside the cube (a stencil
+// clamped at a face, a centre that is not finite, markers not yet placed) is
+// left to the per-node interpolation, so the result is the same bit for bit
+// either way.
 // Box row of a particle, int32[3]: {cache offset in nodes, R (-1: no box),
 // first 256-node chunk of the box in the chunk list}; a box that does not fit
 // the cache's `cap` nodes is not used.
@@ -196,13 +232,34 @@ __device__ inline void box_node(const KParams& p, const typename S::elem* __rest
 // Interpolation + direct forcing (src/funlbm/lbm/lbm3d.py:66-100):
 //   s_u = sum_nodes sum_d u_d w (the three components summed into one scalar),
 //   s_rho = sum w rho, lu = cu + cw x (lx - cx), lF = 2 s_rho (lu - s_u)/dt.
-// FROM_STATE: u', rho' come from the fused state (state_node_moments, or the
-// box cache where box_row and cache are given and the node is in the box);
-// otherwise from the explicit fields u / rho.  Call with all 64 lanes of the
-// wave.
+// interp_marker: u', rho' from the explicit fields u / rho, or (FROM_STATE)
+// pulled out of the fused state per node (state_node_moments).
+// interp_marker_boxed: from the particle's box cache; a marker with a stencil
+// node outside its box is appended to miss_list ([0] = count) instead, for
+// interp_marker to finish.  Both give the same bits.  Call with the 16 lanes
+// of the marker (sub = lane within them).
 // ---------------------------------------------------------------------------
+__device__ inline void interp_finish(int m, int sub, const float x[3], const double* __restrict__ P,
+                                     float su, float sr, float wsum, float dt, bool check_sum,
+                                     float* __restrict__ lu, float* __restrict__ lF,
+                                     int* __restrict__ fail_flag) {
+  if (sub != 0) return;
+  if (check_sum && fabsf(wsum - 1.0f) > 1e-3f) atomicOr(fail_flag, 1);
+  const double d[3] = {(double)x[0] - P[FUNLBM_P_CX], (double)x[1] - P[FUNLBM_P_CX + 1],
+                       (double)x[2] - P[FUNLBM_P_CX + 2]};
+  const double* w = P + FUNLBM_P_CW;
+  const double cr[3] = {w[1] * d[2] - w[2] * d[1], w[2] * d[0] - w[0] * d[2],
+                        w[0] * d[1] - w[1] * d[0]};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float l = (float)(P[FUNLBM_P_CU + a] + cr[a]);
+    lu[3 * m + a] = l;
+    lF[3 * m + a] = ((2.0f * sr) * (l - su)) / dt;
+  }
+}
+
 template <class L, class S, bool FROM_STATE>
-__device__ inline void interp_marker(const KParams& p, int m, int lane,
+__device__ inline void interp_marker(const KParams& p, int m, int sub,
                                      const typename S::elem* __restrict__ state,
                                      const float* __restrict__ fol_old,
                                      const float* __restrict__ rho_face,
@@ -210,55 +267,79 @@ __device__ inline void interp_marker(const KParams& p, int m, int lane,
                                      const float* __restrict__ rhofield,
                                      const float* __restrict__ lx, const double* __restrict__ P,
                                      float dt, bool check_sum, float* __restrict__ lu,
-                                     float* __restrict__ lF, int* __restrict__ fail_flag,
-                                     const int* __restrict__ box_row = nullptr,
-                                     const float4* __restrict__ cache = nullptr, int cap = 0) {
+                                     float* __restrict__ lF, int* __restrict__ fail_flag) {
   const float x[3] = {lx[3 * m], lx[3 * m + 1], lx[3 * m + 2]};
-  const LaneW lw = lane_weight(p, x, lane);
-  float su = 0.0f, sr = 0.0f;
-  if (lw.valid) {
+  const LaneW lw = lane_weight(p, x, sub);
+  float su4[4], sr4[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    su4[k] = 0.0f;
+    sr4[k] = 0.0f;
+  }
+  // one copy of the per-node work, looped over the valid layers
+#pragma unroll 1
+  for (unsigned left = lw.valid; left; left &= left - 1) {
+    const int k = __builtin_ctz(left);
+    const int xx = lw.x0 + k;
     float uu[3], rr;
     if constexpr (FROM_STATE) {
-      bool hit = false;
-      if (cache) {
-        const Box b = box_of(P, box_row, cap);
-        const int i = lw.x - b.o[0], j = lw.y - b.o[1], k = lw.z - b.o[2];
-        hit = b.ok && (unsigned)i < (unsigned)b.S && (unsigned)j < (unsigned)b.S &&
-              (unsigned)k < (unsigned)b.S;
-        if (hit) {
-          const float4 v = cache[b.off + (i * b.S + j) * b.S + k];
-          rr = v.x;
-          uu[0] = v.y;
-          uu[1] = v.z;
-          uu[2] = v.w;
-        }
-      }
-      if (!hit) state_node_moments<L, S>(p, state, fol_old, rho_face, lw.x, lw.y, lw.z, rr, uu);
+      state_node_moments<L, S>(p, state, fol_old, rho_face, xx, lw.y, lw.z, rr, uu);
     } else {
-      const int64_t node = lin(p, lw.x, lw.y, lw.z);
+      const int64_t node = lin(p, xx, lw.y, lw.z);
       rr = rhofield[node];
 #pragma unroll
       for (int d = 0; d < 3; ++d) uu[d] = ufield[(int64_t)d * p.n + node];
     }
-    su = (uu[0] * lw.w + uu[1] * lw.w) + uu[2] * lw.w;
-    sr = rr * lw.w;
-  }
-  su = wave_sum(su);
-  sr = wave_sum(sr);
-  if (lane == 0) {
-    if (check_sum && fabsf(lw.wsum - 1.0f) > 1e-3f) atomicOr(fail_flag, 1);
-    const double d[3] = {(double)x[0] - P[FUNLBM_P_CX], (double)x[1] - P[FUNLBM_P_CX + 1],
-                         (double)x[2] - P[FUNLBM_P_CX + 2]};
-    const double* w = P + FUNLBM_P_CW;
-    const double cr[3] = {w[1] * d[2] - w[2] * d[1], w[2] * d[0] - w[0] * d[2],
-                          w[0] * d[1] - w[1] * d[0]};
+    const float w = k == 0 ? lw.w[0] : k == 1 ? lw.w[1] : k == 2 ? lw.w[2] : lw.w[3];
+    const float a = (uu[0] * w + uu[1] * w) + uu[2] * w, r = rr * w;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-      const float l = (float)(P[FUNLBM_P_CU + a] + cr[a]);
-      lu[3 * m + a] = l;
-      lF[3 * m + a] = ((2.0f * sr) * (l - su)) / dt;
+    for (int q = 0; q < 4; ++q) {
+      if (q == k) {
+        su4[q] = a;
+        sr4[q] = r;
+      }
     }
   }
+  interp_finish(m, sub, x, P, stencil_sum(su4), stencil_sum(sr4), lw.wsum, dt, check_sum, lu, lF,
+                fail_flag);
+}
+
+__device__ inline void interp_marker_boxed(const KParams& p, int m, int sub,
+                                           const float* __restrict__ lx,
+                                           const double* __restrict__ P, float dt, bool check_sum,
+                                           float* __restrict__ lu, float* __restrict__ lF,
+                                           int* __restrict__ fail_flag,
+                                           const int* __restrict__ box_row,
+                                           const float4* __restrict__ cache, int cap,
+                                           int* __restrict__ miss_list) {
+  const float x[3] = {lx[3 * m], lx[3 * m + 1], lx[3 * m + 2]};
+  const LaneW lw = lane_weight(p, x, sub);
+  const Box b = box_of(P, box_row, cap);
+  const int i0 = lw.x0 - b.o[0], j = lw.y - b.o[1], k = lw.z - b.o[2];
+  const bool in_yz = b.ok && (unsigned)j < (unsigned)b.S && (unsigned)k < (unsigned)b.S;
+  bool miss = false;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    miss |= ((lw.valid >> i) & 1u) && !(in_yz && (unsigned)(i0 + i) < (unsigned)b.S);
+  // the marker's 16 lanes decide together
+  const int g = (int)(threadIdx.x & (kWave - 1)) / kMarkerLanes;
+  if ((__ballot(miss) >> (kMarkerLanes * g)) & 0xffffull) {
+    if (sub == 0) miss_list[1 + atomicAdd(&miss_list[0], 1)] = m;
+    return;
+  }
+  float su4[4], sr4[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    su4[i] = 0.0f;
+    sr4[i] = 0.0f;
+    if (!((lw.valid >> i) & 1u)) continue;
+    const float4 v = cache[b.off + ((i0 + i) * b.S + j) * b.S + k];
+    const float w = lw.w[i];
+    su4[i] = (v.y * w + v.z * w) + v.w * w;
+    sr4[i] = v.x * w;
+  }
+  interp_finish(m, sub, x, P, stencil_sum(su4), stencil_sum(sr4), lw.wsum, dt, check_sum, lu, lF,
+                fail_flag);
 }
 
 // ---------------------------------------------------------------------------
@@ -483,32 +564,100 @@ __device__ inline void pose_particle(int b, int e, const float* __restrict__ lag
 //          and raise owner[node] to (stamp, marker) with an integer atomicMax
 //   write  the owning marker of each node writes w lF lm and stamps the tile
 // The stamp advances by one per spreading, so owner[] never needs clearing.
+// The four markers of a wavefront are consecutive and their stencils overlap:
+// prep leaves a node that a later marker of the wave (of the same lattice)
+// also claims, or also clears, to that marker, so the wave issues each claim
+// and each clear once.  The atomicMax result is unchanged (the later key is
+// the larger one).
 // ---------------------------------------------------------------------------
-__device__ inline void spread_prep_marker(const KParams& p, int m, int key_id, int lane,
-                                          const float* __restrict__ lx,
+struct PrepPeer {
+  int lo[3];  // stencil origin (claims)
+  int ol[3];  // last stencil origin of this buffer (clears)
+  int flags;  // bit 0: active, bit 1: has a stencil to clear
+  int tag;    // lattice of the marker
+};
+
+__device__ inline bool in_box4(const int lo[3], int x, int y, int z, int ex, int ey, int ez) {
+  // [lo, min(lo + 4, e)) per axis
+  return x >= lo[0] && x < min(lo[0] + 4, ex) && y >= lo[1] && y < min(lo[1] + 4, ey) && z >= lo[2] &&
+         z < min(lo[2] + 4, ez);
+}
+
+// Call with every lane of the wave (inactive markers included: active =
+// false), the wave's four markers consecutive.  peers: this wave's four
+// entries of block-shared memory.
+__device__ inline void spread_prep_marker(const KParams& p, bool active, int tag, int m, int key_id,
+                                          int sub, const float* __restrict__ lx,
                                           const int* __restrict__ clear_origin,
                                           float* __restrict__ fol,
-                                          unsigned long long* __restrict__ owner, uint32_t stamp) {
-  const int i = lane >> 4, j = (lane >> 2) & 3, k = lane & 3;
-  if (clear_origin) {
-    const int ox = clear_origin[3 * m] + i, oy = clear_origin[3 * m + 1] + j,
-              oz = clear_origin[3 * m + 2] + k;
-    if (ox < p.nx && oy < p.ny && oz < p.nz) {
-      const int64_t node = lin(p, ox, oy, oz);
+                                          unsigned long long* __restrict__ owner, uint32_t stamp,
+                                          PrepPeer* peers) {
+  const int g = (int)(threadIdx.x & (kWave - 1)) / kMarkerLanes;
+  const int j = sub >> 2, k = sub & 3;
+  Stencil s{};
+  int ol[3] = {0, 0, 0};
+  const bool clear = active && clear_origin != nullptr;
+  if (active) {
+    const float x[3] = {lx[3 * m], lx[3 * m + 1], lx[3 * m + 2]};
+    s = stencil_of(p, x);
+  }
+  if (clear) {
 #pragma unroll
-      for (int d = 0; d < 3; ++d) fol[(int64_t)d * p.n + node] = 0.0f;
+    for (int d = 0; d < 3; ++d) ol[d] = clear_origin[3 * m + d];
+  }
+  if (sub == 0) {
+    PrepPeer& e = peers[g];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      e.lo[d] = s.lo[d];
+      e.ol[d] = ol[d];
+    }
+    e.flags = (active ? 1 : 0) | (clear ? 2 : 0);
+    e.tag = tag;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (!active) return;
+  const int size[3] = {p.nx - 1, p.ny - 1, p.nz - 1};
+  if (clear) {
+    const int oy = ol[1] + j, oz = ol[2] + k;
+    if (oy < p.ny && oz < p.nz) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int ox = ol[0] + i;
+        if (ox >= p.nx) break;
+        bool later = false;
+        for (int q = g + 1; q < kMarkersPerWave; ++q) {
+          const PrepPeer& e = peers[q];
+          later |= (e.flags & 2) && e.tag == tag && in_box4(e.ol, ox, oy, oz, p.nx, p.ny, p.nz);
+        }
+        if (later) continue;
+        const int64_t node = lin(p, ox, oy, oz);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) fol[(int64_t)d * p.n + node] = 0.0f;
+      }
     }
   }
-  const float x[3] = {lx[3 * m], lx[3 * m + 1], lx[3 * m + 2]};
-  const Stencil s = stencil_of(p, x);
-  const int nx_ = s.lo[0] + i, ny_ = s.lo[1] + j, nz_ = s.lo[2] + k;
-  if (nx_ < s.hi[0] && ny_ < s.hi[1] && nz_ < s.hi[2]) {
+  const int ny_ = s.lo[1] + j, nz_ = s.lo[2] + k;
+  if (ny_ < s.hi[1] && nz_ < s.hi[2]) {
     const unsigned long long key = ((unsigned long long)stamp << 32) | (unsigned long long)(unsigned)key_id;
-    atomicMax(&owner[lin(p, nx_, ny_, nz_)], key);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int nx_ = s.lo[0] + i;
+      if (nx_ >= s.hi[0]) break;
+      bool later = false;
+      for (int q = g + 1; q < kMarkersPerWave; ++q) {
+        const PrepPeer& e = peers[q];
+        later |= (e.flags & 1) && e.tag == tag && in_box4(e.lo, nx_, ny_, nz_, size[0], size[1], size[2]);
+      }
+      if (later) continue;
+      atomicMax(&owner[lin(p, nx_, ny_, nz_)], key);
+    }
   }
 }
 
-__device__ inline void spread_write_marker(const KParams& p, int m, int key_id, int lane,
+__device__ inline void spread_write_marker(const KParams& p, int m, int key_id, int sub,
                                            const float* __restrict__ lx,
                                            const float* __restrict__ lF,
                                            const float* __restrict__ lm, float* __restrict__ fol,
@@ -516,21 +665,26 @@ __device__ inline void spread_write_marker(const KParams& p, int m, int key_id, 
                                            int* __restrict__ origin, uint32_t* __restrict__ tiles,
                                            uint32_t stamp) {
   const float x[3] = {lx[3 * m], lx[3 * m + 1], lx[3 * m + 2]};
-  const LaneW lw = lane_weight(p, x, lane);
-  if (lane == 0 && origin) {
+  const LaneW lw = lane_weight(p, x, sub);
+  if (sub == 0 && origin) {
     const Stencil s = stencil_of(p, x);
     origin[3 * m] = s.lo[0];
     origin[3 * m + 1] = s.lo[1];
     origin[3 * m + 2] = s.lo[2];
   }
   if (!lw.valid) return;
-  const int64_t node = lin(p, lw.x, lw.y, lw.z);
   const unsigned long long key = ((unsigned long long)stamp << 32) | (unsigned long long)(unsigned)key_id;
-  if (owner[node] != key) return;
+  const float f[3] = {lF[3 * m], lF[3 * m + 1], lF[3 * m + 2]};
   const float mm = lm[m];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) fol[(int64_t)d * p.n + node] = (lw.w * lF[3 * m + d]) * mm;
-  if (tiles) tiles[(uint32_t)node >> 8] = stamp;
+  for (int i = 0; i < 4; ++i) {
+    if (!((lw.valid >> i) & 1u)) continue;
+    const int64_t node = lin(p, lw.x0 + i, lw.y, lw.z);
+    if (owner[node] != key) continue;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) fol[(int64_t)d * p.n + node] = (lw.w[i] * f[d]) * mm;
+    if (tiles) tiles[(uint32_t)node >> 8] = stamp;
+  }
 }
 
 }  // namespace funlbm

@@ -46,6 +46,7 @@ class LBMD3(LBMBase):
             self._box_chunks = torch.from_numpy(chunks).to(dev) if len(chunks) else None
             self._box_cache = torch.empty(4 * max(nodes, 1), device=dev, dtype=torch.float32)
             self._box_nodes = nodes
+            self._box_miss = torch.zeros(sw.n_markers + 1, device=dev, dtype=torch.int32)
 
     # ------------------------------------------------------------------
     # IBM phases
@@ -59,7 +60,7 @@ class LBMD3(LBMBase):
                 "funlbm_ibm_interp_boxed", fl.lat, nat.ptr(state), nat.ptr(fl.fol_old()), nat.ptr(rface),
                 sw.n_markers, nat.ptr(sw.lx), nat.ptr(sw.pid), len(sw.particles), nat.ptr(sw.pstate),
                 nat.ptr(self._box_rows), nch, nat.ptr(self._box_chunks) if nch else None,
-                nat.ptr(self._box_cache), self._box_nodes, float(self.config.dt),
+                nat.ptr(self._box_cache), self._box_nodes, nat.ptr(self._box_miss), float(self.config.dt),
                 1 if self.step > 100 else 0, nat.ptr(sw.lu), nat.ptr(sw.lF), nat.ptr(sw.ctl),
                 fl._stream(),
             )

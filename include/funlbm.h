@@ -239,15 +239,18 @@ int funlbm_ibm_interp(const funlbm_lattice* lat, const void* state, const float*
  * (its markers pull per node).  chunk_particle[n_chunks] names the particle of
  * each 256-node chunk, ceil((2R + 4)^3 / 256) consecutive chunks per box from
  * its first chunk.  box_cache holds cache_nodes float4 (16-byte aligned); a box
- * beyond it is not used.  The cache is scratch: its contents after the call
- * are the boxes' moments of this state. */
+ * beyond it is not used.  A marker with a stencil node outside its box (no
+ * box, a stencil clamped at a face, markers not yet placed) is interpolated
+ * per node after the others: miss_list (int32[n_markers + 1]) lists them.
+ * The cache and the list are scratch.  n_chunks = 0: every marker per node
+ * (box_cache, chunk_particle and miss_list may then be NULL). */
 int funlbm_ibm_interp_boxed(const funlbm_lattice* lat, const void* state, const float* fol_old,
                             const float* rho_face, int32_t n_markers, const float* lx,
                             const int32_t* marker_particle, int32_t n_particles,
                             const double* pstate, const int32_t* particle_box, int32_t n_chunks,
                             const int32_t* chunk_particle, float* box_cache, int32_t cache_nodes,
-                            float dt, int32_t check_weight_sum, float* lu, float* lF, int32_t* ctl,
-                            void* stream);
+                            int32_t* miss_list, float dt, int32_t check_weight_sum, float* lu,
+                            float* lF, int32_t* ctl, void* stream);
 
 /* Wall spring, replaces LBMD3.particle_to_wall (src/funlbm/lbm/lbm3d.py:121-167). */
 int funlbm_wall_spring(const funlbm_lattice* lat, int32_t n_particles, const int32_t* marker_offsets,
@@ -322,9 +325,10 @@ int funlbm_case_pack(const funlbm_lattice* lat, int64_t f_offset, int64_t node_o
  * holds [begin, end) marker ranges (two int32 per particle).  traj (may be
  * NULL) receives float[4] = (cx, cy, cz, step) per particle and step in a ring
  * of traj_cap[0] entries.  box_cache (may be NULL: every marker pulls per
- * node), particle_box (int32[3] per particle slot) and chunk_particle are the
- * moment boxes of funlbm_ibm_interp_boxed, over particle slots.  The grid
- * sizes fix the persistent launch shapes. */
+ * node), particle_box (int32[3] per particle slot), chunk_particle and
+ * miss_list (int32, marker slots + 1) are the moment boxes of
+ * funlbm_ibm_interp_boxed, over particle slots.  The grid sizes fix the
+ * persistent launch shapes. */
 int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, int32_t* case_par,
                       int32_t* case_left, int32_t* case_done, int32_t* case_flag,
                       const int32_t* tile_case, const int32_t* tile_local,
@@ -337,7 +341,7 @@ int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, in
                       const int32_t* traj_cap, int32_t grid_tiles, int32_t grid_markers,
                       int32_t grid_particles, int32_t with_ibm, const int32_t* particle_box,
                       const int32_t* chunk_particle, float* box_cache, int32_t cache_nodes,
-                      void* stream);
+                      int32_t* miss_list, void* stream);
 
 /* min, mean and max of the reference's f, u and rho after the last fused
  * step, computed from the state without materialising the fields (the

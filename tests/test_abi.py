@@ -178,7 +178,7 @@ def test_boxed_interp_argument_errors_need_no_device():
     args = lambda **kw: [kw.get(k, v) for k, v in (  # noqa: E731
         ("lat", lp), ("state", p), ("fol", p), ("rface", None), ("nm", 10), ("lx", p), ("pid", p),
         ("np", 1), ("ps", p), ("box", p), ("nch", 4), ("chunks", p), ("cache", p), ("cap", 1024),
-        ("dt", 1.0), ("check", 0), ("lu", p), ("lF", p), ("ctl", p), ("stream", None))]
+        ("miss", p), ("dt", 1.0), ("check", 0), ("lu", p), ("lF", p), ("ctl", p), ("stream", None))]
     f = L.funlbm_ibm_interp_boxed
     assert f(*args(nm=0)) == 0  # nothing to do: no launch
     assert f(*args(nm=-1)) == nat.EINVAL
@@ -187,6 +187,7 @@ def test_boxed_interp_argument_errors_need_no_device():
     assert f(*args(state=None)) == nat.EINVAL
     assert f(*args(box=None)) == nat.EINVAL
     assert f(*args(chunks=None)) == nat.EINVAL
+    assert f(*args(miss=None)) == nat.EINVAL
     assert f(*args(cache=ctypes.c_void_p(4096 + 4))) == nat.EINVAL
     assert "aligned" in nat.last_error()
     assert f(*args(dt=0.0)) == nat.EINVAL

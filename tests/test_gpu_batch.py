@@ -94,6 +94,30 @@ def test_alone_equals_mixed_bitwise(rounding):
             assert torch.equal(alone.particle_rows(a)[:, :15], mix.particle_rows(cid)[:, :15])
 
 
+def test_cases_without_moment_boxes_equal_boxed_ones():
+    """A cache too small for any box (every marker interpolated per node, from
+    the miss list) gives the bits of the boxed interpolation; so does a cache
+    that holds only some of the boxes."""
+    specs = _mixed_specs()
+    runs = []
+    for cap in (None, 0, 3000):
+        eng = BatchEngine(storage="f16", max_nodes=1 << 17, max_markers=8192, max_particles=16,
+                          max_cases=8, max_box_nodes=cap)
+        ids = [eng.add(s) for s in specs]
+        if cap == 0:
+            assert all(R < 0 for s in eng.slots.values() for _, R in s.boxes)
+        if cap == 3000:
+            radii = [R for s in eng.slots.values() for _, R in s.boxes]
+            assert min(radii) < 0 <= max(radii)
+        eng.step(40)
+        runs.append([(eng.fields(c), eng.particle_rows(c)[:, :15] if s.particles else None)
+                     for s, c in zip(specs, ids)])
+    for other in runs[1:]:
+        for ((u0, r0), p0), ((u1, r1), p1) in zip(runs[0], other):
+            assert torch.equal(u0, u1) and torch.equal(r0, r1)
+            assert (p0 is None and p1 is None) or torch.equal(p0, p1)
+
+
 def test_graph_replay_equals_eager_and_step_counts():
     specs = _mixed_specs()
     specs[1].steps = 5
