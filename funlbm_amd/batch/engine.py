@@ -158,7 +158,9 @@ class BatchEngine:
         self.dev_scale = float(dev_scale)
         self.max_cases = int(max_cases)
         self.traj_cap = int(traj_cap)
-        elem_cap = int(max_nodes) * int(max_q)
+        # a case's state planes hold whole tiles plus two slack tiles (up to
+        # 3 * 4096 nodes beyond its own)
+        elem_cap = (int(max_nodes) + 3 * 4096 * int(max_cases)) * int(max_q)
         tile_cap = int(max_nodes) // TILE + int(max_cases)
         self.elems = RangeAllocator(elem_cap, align=64)
         self.nodes = RangeAllocator(int(max_nodes), align=TILE)

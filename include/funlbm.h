@@ -87,7 +87,7 @@ int funlbm_abi_version(void);
 const char* funlbm_last_error(void);
 
 /* Bytes of one fused-state buffer for `lat`.  The nodes are grouped into
- * tiles of T consecutive nodes (T = 4096 from 2^20 nodes up, else 256); a tile
+ * tiles of T consecutive nodes (T = 4096 from 2^14 nodes up, else 256); a tile
  * holds its q direction planes of T elements one after another, and one slack
  * tile precedes the first tile and one follows the last:
  * (ceil(n / T) + 2) * T * q elements of lat->storage.  Element (k, node j)

@@ -135,7 +135,7 @@ def rne_round16(d, storage):
 
 
 def state_tile_shift(n):
-    return 12 if n >= (1 << 20) else 8
+    return 12 if n >= (1 << 14) else 8
 
 
 def state_elems(q, n):

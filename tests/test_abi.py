@@ -121,8 +121,8 @@ def test_marker_wrapper_rejects_bad_shapes_before_any_device_call():
         surface_markers([(1.0, 1.0, 1.0)], device="cpu")
 
 
-@pytest.mark.parametrize("dims,storage", [((4, 4, 4), 0), ((7, 9, 11), 1), ((64, 64, 64), 0), ((256, 256, 256), 1),
-                                          ((255, 255, 255), 0)])
+@pytest.mark.parametrize("dims,storage", [((4, 4, 4), 0), ((7, 9, 11), 1), ((25, 25, 26), 1), ((16, 32, 32), 0),
+                                          ((64, 64, 64), 0), ((256, 256, 256), 1), ((255, 255, 255), 0)])
 def test_state_plane_has_slack_and_is_staggered(dims, storage):
     """A state buffer holds whole tiles of the node count plus two slack
     tiles (the z-face corner loads overreach by one node): more than n
@@ -139,7 +139,7 @@ def test_state_plane_has_slack_and_is_staggered(dims, storage):
     n = dims[0] * dims[1] * dims[2]
     plane = nat.state_plane(lat)
     assert plane > n
-    tile = 4096 if n >= 1 << 20 else 256
+    tile = 4096 if n >= 1 << 14 else 256
     assert plane == (-(-n // tile) + 2) * tile
 
 

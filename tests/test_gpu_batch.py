@@ -42,7 +42,7 @@ def lbmd3_for(spec, storage, rounding="nearest"):
 def test_batch_case_equals_single_lattice(storage, rounding):
     """With stochastic rounding the case's dither follows its own step count
     from its seed, as the single-lattice solve's does."""
-    spec = sphere_case((20, 22, 26), steps=25)
+    spec = sphere_case((26, 22, 30), steps=25)  # 17160 nodes: 4096-node state tiles
     spec.seed = 99
     lbm = lbmd3_for(spec, storage, rounding)
     spec.tau = lbm.flow.lat.tau  # the solver's tau comes from mu; use the same float32
@@ -62,11 +62,11 @@ def test_batch_case_equals_single_lattice(storage, rounding):
 
 def _mixed_specs():
     return [
-        sphere_case((24, 20, 30), q=19, tau=0.62, r=4.0, steps=40),
+        sphere_case((26, 24, 30), q=19, tau=0.62, r=4.0, steps=40),  # 4096-node state tiles
         CaseSpec(nodes=(18, 18, 18), q=15, tau=0.8, steps=40,
                  particles=[ParticleSpec(kind="sphere", r=3.0, rou=2.0, center=(9, 9, 10))]),
-        CaseSpec(nodes=(33, 17, 21), q=27, tau=0.55, steps=40,
-                 particles=[ParticleSpec(kind="ellipsoid", abc=(3, 3, 5), rou=1.3, center=(16, 8, 10),
+        CaseSpec(nodes=(33, 25, 21), q=27, tau=0.55, steps=40,
+                 particles=[ParticleSpec(kind="ellipsoid", abc=(3, 3, 5), rou=1.3, center=(16, 12, 10),
                                          angle=(0.3, 0.2, 0.1)),
                             ParticleSpec(kind="sphere", r=2.5, rou=2.5, center=(8, 8, 12))]),
         CaseSpec(nodes=(16, 16, 40), q=19, tau=0.9, steps=40, particles=[]),
