@@ -17,12 +17,20 @@ from ..particle import boxes
 from .base import LBMBase
 
 
+# markers from which the boxed interpolation beats the per-node one: below,
+# the per-node kernel takes a few microseconds and the box pass and the
+# fallback pass are two more launches
+BOXED_MIN_MARKERS = 4096
+
+
 class LBMD3(LBMBase):
     def __init__(self, *args, **kwargs):
         super().__init__(*args, **kwargs)
         # interpolation from the fused state through the particles' moment
-        # boxes (same results as the per-node pull, which False selects)
-        self.boxed_interp = True
+        # boxes (same results as the per-node pull, which False selects);
+        # None: boxes from BOXED_MIN_MARKERS markers up, where they pay for
+        # their two extra launches
+        self.boxed_interp = None
 
     def _init(self):
         """flow.init, one extra collision, particles (src/funlbm/lbm/lbm3d.py:23-30)."""
@@ -53,7 +61,10 @@ class LBMD3(LBMBase):
     # ------------------------------------------------------------------
     def _interp(self, from_state: bool):
         sw, fl = self.particle_swarm, self.flow
-        if from_state and self.boxed_interp:
+        boxed = self.boxed_interp
+        if boxed is None:
+            boxed = sw.n_markers >= BOXED_MIN_MARKERS
+        if from_state and boxed:
             state, rface = fl.state_source()
             nch = 0 if self._box_chunks is None else int(self._box_chunks.numel())
             nat.call(
