@@ -165,8 +165,8 @@ __device__ __forceinline__ void state_node_moments(const KParams& p,
 
 // ---------------------------------------------------------------------------
 // Moment cache of a particle's box.  The markers of a particle lie within R
-// of its centre cx (R: the ceiling of its largest body-frame marker distance,
-// plus a margin for float rounding), so a stencil node n of a marker x, with
+// of its centre cx (R: the ceiling of its largest body-frame marker distance),
+// so a stencil node n of a marker x, with
 // floor(x) - 1 <= n <= floor(x) + 2 and floor(cx) - R <= floor(x) <=
 // floor(cx) + R, lies in the cube [o, o + S)^3 with o = floor(cx) - (R + 1)
 // and S = 2R + 4.  One pass (box_node) stores (rho, ux, uy, uz) of every cube

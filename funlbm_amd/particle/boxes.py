@@ -25,13 +25,15 @@ NODES_PER_MARKER = 32
 
 def box_radius(lag: np.ndarray) -> int:
     """R for body-frame markers `lag` ([M, 3]): the ceiling of their largest
-    distance from the centre, with a margin for the float rounding of the
-    marker positions; -1 if the box is not worth computing."""
+    distance from the centre (a marker that float rounding puts a hair
+    beyond it misses its box only while a centre coordinate lies within that
+    hair of an integer, and is then interpolated per node); -1 if the box is
+    not worth computing."""
     lag = np.asarray(lag, dtype=np.float64).reshape(-1, 3)
     if len(lag) == 0:
         return -1
     r = float(np.sqrt((lag * lag).sum(1)).max())
-    R = int(math.ceil(r * (1.0 + 1e-6) + 1e-3))
+    R = max(int(math.ceil(r - 1e-4)), 0)
     if R > MAX_R or box_nodes(R) > NODES_PER_MARKER * len(lag):
         return -1
     return R
