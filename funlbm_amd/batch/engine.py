@@ -212,6 +212,8 @@ class BatchEngine:
         self.grid = {"tiles": 2048, "markers": 1024, "particles": 256}
         self._graph = None
         self._graph_steps = 0
+        self._graph_mask = 0
+        self._mask = 0  # lattices with cases (bit i: D3Q13, 15, 19, 27)
         self._dirty = True
 
     # ------------------------------------------------------------------ admission
@@ -350,6 +352,8 @@ class BatchEngine:
                     tl.append(np.arange(s.n_tiles, dtype=np.int32))
             ends.append(int(sum(len(a) for a in tc)))
         nt = ends[-1]
+        self._mask = sum(1 << i for i, q in enumerate((13, 15, 19, 27))
+                         if any(s.spec.q == q for s in self.slots.values()))
         if nt:
             self.tile_case[:nt] = torch.from_numpy(np.concatenate(tc)).to(self.device)
             self.tile_local[:nt] = torch.from_numpy(np.concatenate(tl)).to(self.device)
@@ -383,7 +387,8 @@ class BatchEngine:
             P(self.origin[1]), P(self.origin_ok), P(self.lag), P(self.lx), P(self.lu), P(self.lF),
             P(self.lm), P(self.pstate), P(self.traj), P(self.traj_cap_t), int(g["tiles"]),
             int(g["markers"]), int(g["particles"]), 1, P(self.particle_box), P(self.chunk_particle),
-            P(self.box_cache), self.box_cap, P(self.box_miss), nat.stream_handle(self.device),
+            P(self.box_cache), self.box_cap, P(self.box_miss), self._mask,
+            nat.stream_handle(self.device),
         )
 
     def step(self, n: int = 1):
@@ -437,11 +442,15 @@ class BatchEngine:
                 for _ in range(n):
                     self._launch()
         torch.cuda.current_stream(self.device).wait_stream(s)
-        self._graph, self._graph_steps = g, n
+        self._graph, self._graph_steps, self._graph_mask = g, n, self._mask
         return g
 
     def run_graph(self):
+        """Replays the captured steps; captures again first when a case of a
+        lattice the graph does not launch has been admitted since."""
         self._publish()
+        if self._mask & ~self._graph_mask:
+            self.capture(self._graph_steps)
         self._graph.replay()
 
     # ------------------------------------------------------------------ results

@@ -364,7 +364,7 @@ int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, in
                       const int32_t* traj_cap, int32_t grid_tiles, int32_t grid_markers,
                       int32_t grid_particles, int32_t with_ibm, const int32_t* particle_box,
                       const int32_t* chunk_particle, float* box_cache, int32_t cache_nodes,
-                      int32_t* miss_list, void* stream) {
+                      int32_t* miss_list, int32_t lattice_mask, void* stream) {
   if (!cases || !hdr || !case_par || !case_left || !case_done || !case_flag || !tile_case ||
       !tile_local || !pool_a || !pool_b || !fol_a || !fol_b || !fol_tiles)
     return set_error(FUNLBM_EINVAL, "batch_step: NULL array");
@@ -428,7 +428,9 @@ int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, in
                          (const float*)fol_b, (const uint32_t*)fol_tiles);
       return check_launch("batch_fused");
     };
-    if ((rc = fused(D3Q13{})) || (rc = fused(D3Q15{})) || (rc = fused(D3Q19{})) || (rc = fused(D3Q27{})))
+    // only the lattices that have cases (an empty launch still sweeps its grid)
+    if (((lattice_mask & 1) && (rc = fused(D3Q13{}))) || ((lattice_mask & 2) && (rc = fused(D3Q15{}))) ||
+        ((lattice_mask & 4) && (rc = fused(D3Q19{}))) || ((lattice_mask & 8) && (rc = fused(D3Q27{}))))
       return rc;
     if (with_ibm) {
       hipLaunchKernelGGL(k_batch_pose, dim3(grid_particles), dim3(kIbmBlock), 0, s, hdr, dyn,

@@ -327,8 +327,10 @@ int funlbm_case_pack(const funlbm_lattice* lat, int64_t f_offset, int64_t node_o
  * of traj_cap[0] entries.  box_cache (may be NULL: every marker pulls per
  * node), particle_box (int32[3] per particle slot), chunk_particle and
  * miss_list (int32, marker slots + 1) are the moment boxes of
- * funlbm_ibm_interp_boxed, over particle slots.  The grid sizes fix the
- * persistent launch shapes. */
+ * funlbm_ibm_interp_boxed, over particle slots.  lattice_mask (bits 0-3:
+ * D3Q13, D3Q15, D3Q19, D3Q27) selects the fused kernels launched; a lattice
+ * whose bit is clear is not stepped.  The grid sizes fix the persistent
+ * launch shapes. */
 int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, int32_t* case_par,
                       int32_t* case_left, int32_t* case_done, int32_t* case_flag,
                       const int32_t* tile_case, const int32_t* tile_local,
@@ -341,7 +343,7 @@ int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, in
                       const int32_t* traj_cap, int32_t grid_tiles, int32_t grid_markers,
                       int32_t grid_particles, int32_t with_ibm, const int32_t* particle_box,
                       const int32_t* chunk_particle, float* box_cache, int32_t cache_nodes,
-                      int32_t* miss_list, void* stream);
+                      int32_t* miss_list, int32_t lattice_mask, void* stream);
 
 /* min, mean and max of the reference's f, u and rho after the last fused
  * step, computed from the state without materialising the fields (the

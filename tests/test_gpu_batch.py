@@ -141,6 +141,35 @@ def test_graph_replay_equals_eager_and_step_counts():
         assert torch.equal(ua, ub) and torch.equal(ra, rb)
 
 
+def test_graph_follows_a_lattice_admitted_after_capture():
+    """Only the lattices with cases are launched; a graph captured before a
+    case of another lattice arrives is captured again, and the new case
+    steps as it would eagerly."""
+    specs = _mixed_specs()
+    first = [s for s in specs if s.q == 19]
+    late = [s for s in specs if s.q != 19]
+    eager = BatchEngine(storage="f32", max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8)
+    graph = BatchEngine(storage="f32", max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8)
+    ie = [eager.add(s) for s in first]
+    ig = [graph.add(s) for s in first]
+    eager.step(4)
+    graph.capture(4)
+    graph.run_graph()
+    assert graph._graph_mask == 1 << 2
+    ie += [eager.add(s) for s in late]
+    ig += [graph.add(s) for s in late]
+    eager.step(8)
+    graph.run_graph()
+    graph.run_graph()
+    torch.cuda.synchronize()
+    assert graph._graph_mask == (1 << 1) | (1 << 2) | (1 << 3)
+    assert graph.case_done.cpu().numpy()[ig].tolist() == eager.case_done.cpu().numpy()[ie].tolist()
+    for a, b in zip(ie, ig):
+        ua, ra = eager.fields(a)
+        ub, rb = graph.fields(b)
+        assert torch.equal(ua, ub) and torch.equal(ra, rb)
+
+
 def test_retire_and_reuse_slot():
     specs = _mixed_specs()
     eng = BatchEngine(storage="f16", max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8)
