@@ -49,17 +49,6 @@ __device__ inline Stencil stencil_of(const KParams& p, const float x[3]) {
   return s;
 }
 
-// Cosine kernel factor (src/funlbm/lbm/lbm3d.py:46-59):
-// phi(r) = (1 + cos(|r pi/2|))/4 where all |r_d| <= 2, else 0.
-__device__ inline float unnormalised_weight(const float x[3], int nx_, int ny_, int nz_) {
-  const float r[3] = {(float)nx_ - x[0], (float)ny_ - x[1], (float)nz_ - x[2]};
-  if (!(fabsf(r[0]) <= 2.0f && fabsf(r[1]) <= 2.0f && fabsf(r[2]) <= 2.0f)) return 0.0f;
-  float t[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) t[d] = (1.0f + cosf(fabsf(r[d] * kPiF / 2.0f))) / 4.0f;
-  return (t[0] * t[1]) * t[2];
-}
-
 // Sum over the 16 lanes of one marker (xor butterfly 8, 4, 2, 1 on lane
 // 4j + k: j's bits first, then k's).
 template <class T>
@@ -79,7 +68,10 @@ __device__ inline T stencil_sum(const T v[4]) {
 }
 
 // Lane geometry of a marker: node (x0 + i, y, z) of each layer i, which of
-// them lie in the stencil, and their normalised weights.
+// them lie in the stencil, and their normalised weights.  The cosine kernel
+// (src/funlbm/lbm/lbm3d.py:46-59): phi(r) = (1 + cos(|r pi/2|))/4 per axis,
+// the node's weight (t_x t_y) t_z where all |r_d| <= 2, else 0, r = node - x;
+// the weights of the stencil are normalised to sum 1.
 struct LaneW {
   int x0, y, z;
   unsigned valid;  // bit i: node i inside the stencil
@@ -95,7 +87,7 @@ __device__ inline LaneW lane_weight(const KParams& p, const float x[3], int sub)
   lw.y = s.lo[1] + j;
   lw.z = s.lo[2] + k;
   const bool vyz = lw.y < s.hi[1] && lw.z < s.hi[2];
-  // unnormalised_weight per node: (t_x t_y) t_z, t_y and t_z shared by the layers
+  // unnormalised weight per node: (t_x t_y) t_z, t_y and t_z shared by the layers
   const float ry = (float)lw.y - x[1], rz = (float)lw.z - x[2];
   const bool inyz = fabsf(ry) <= 2.0f && fabsf(rz) <= 2.0f;
   const float ty = (1.0f + cosf(fabsf(ry * kPiF / 2.0f))) / 4.0f;
