@@ -556,7 +556,10 @@ __device__ inline void pose_particle(int b, int e, const float* __restrict__ lag
 //          and raise owner[node] to (stamp, marker) with an integer atomicMax
 //   write  the owning marker of each node writes w lF lm and stamps the tile
 // The stamp advances by one per spreading, so owner[] never needs clearing.
-// The four markers of a wavefront are consecutive and their stencils overlap:
+// A marker whose stencil origin is the one this buffer received it at clears
+// nothing: the stencil is the same node set, and every node of it is claimed
+// and written again.  The four markers of a wavefront are consecutive and
+// their stencils overlap:
 // prep leaves a node that a later marker of the wave (of the same lattice)
 // also claims, or also clears, to that marker, so the wave issues each claim
 // and each clear once.  The atomicMax result is unchanged (the later key is
@@ -588,7 +591,7 @@ __device__ inline void spread_prep_marker(const KParams& p, bool active, int tag
   const int j = sub >> 2, k = sub & 3;
   Stencil s{};
   int ol[3] = {0, 0, 0};
-  const bool clear = active && clear_origin != nullptr;
+  bool clear = active && clear_origin != nullptr;
   if (active) {
     const float x[3] = {lx[3 * m], lx[3 * m + 1], lx[3 * m + 2]};
     s = stencil_of(p, x);
@@ -596,6 +599,9 @@ __device__ inline void spread_prep_marker(const KParams& p, bool active, int tag
   if (clear) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) ol[d] = clear_origin[3 * m + d];
+    // an unchanged stencil needs no clearing: each of its nodes is claimed
+    // again and written by its owner in this spreading
+    clear = ol[0] != s.lo[0] || ol[1] != s.lo[1] || ol[2] != s.lo[2];
   }
   if (sub == 0) {
     PrepPeer& e = peers[g];
