@@ -223,7 +223,8 @@ __global__ __launch_bounds__(kIbmBlock) void k_batch_spread_prep(
     const int* __restrict__ marker_case, const int* __restrict__ case_marker_begin,
     const float* __restrict__ lx, const int* __restrict__ origin_a, const int* __restrict__ origin_b,
     const int* __restrict__ origin_ok, float* __restrict__ fol_a, float* __restrict__ fol_b,
-    unsigned long long* __restrict__ owner) {
+    unsigned long long* __restrict__ owner, const int* __restrict__ marker_particle,
+    const int* __restrict__ pbox) {
   __shared__ PrepPeer peers[kIbmBlock / kWave][kMarkersPerWave];
   const int nm = hdr[H_MARKERS];
   const uint32_t stamp = (uint32_t)hdr[H_STAMP] + 1u;
@@ -239,7 +240,8 @@ __global__ __launch_bounds__(kIbmBlock) void k_batch_spread_prep(
     bool act = m < nm;
     if (act) {
       c = marker_case[m];
-      act = c >= 0 && dyn.left[c] > 0;
+      // the markers of a particle k_batch_spread_claim serves are left out
+      act = c >= 0 && dyn.left[c] > 0 && !(pbox && claims_in_box(pbox + 3 * marker_particle[m]));
     }
     const CaseDesc& cd = cases[act ? c : 0];
     float* fn = nullptr;
@@ -255,6 +257,35 @@ __global__ __launch_bounds__(kIbmBlock) void k_batch_spread_prep(
       key = m - case_marker_begin[c];
     }
     spread_prep_marker(cd.p, act, c, m, key, sub, lx, org, fn, owner + cd.node_off, stamp, peers[wv]);
+  }
+}
+
+// Claims of the active particles with a box of at most kClaimNodes nodes,
+// one block per particle (spread_claim_particle).
+__global__ __launch_bounds__(kClaimBlock) void k_batch_spread_claim(
+    const CaseDesc* __restrict__ cases, const int* __restrict__ hdr, BatchDyn dyn,
+    const int* __restrict__ particle_case, const int* __restrict__ pm,
+    const int* __restrict__ case_marker_begin, const float* __restrict__ lx,
+    const double* __restrict__ ps, const int* __restrict__ pbox, const int* __restrict__ origin_a,
+    const int* __restrict__ origin_b, const int* __restrict__ origin_ok, float* __restrict__ fol_a,
+    float* __restrict__ fol_b, unsigned long long* __restrict__ owner) {
+  __shared__ int own[kClaimNodes];
+  const int np = hdr[H_PARTICLES];
+  const uint32_t stamp = (uint32_t)hdr[H_STAMP] + 1u;
+  for (int i = blockIdx.x; i < np; i += gridDim.x) {
+    const int c = particle_case[i];
+    if (c < 0 || dyn.left[c] <= 0) continue;
+    const int* row = pbox + 3 * i;
+    if (!claims_in_box(row)) continue;
+    const CaseDesc& cd = cases[c];
+    const int par = dyn.par[c];
+    float* fn = (par ? fol_a : fol_b) + 3 * cd.node_off;
+    const bool ok = (origin_ok[c] >> (1 - par)) & 1;
+    const int* org = ok ? (par ? origin_a : origin_b) : nullptr;
+    const int b = pm[2 * i], e = pm[2 * i + 1];
+    spread_claim_particle(cd.p, b, e, b - case_marker_begin[c], lx, pst(ps, i), row, org, fn,
+                          owner + cd.node_off, stamp, own);
+    __syncthreads();  // own is reused by the next particle
   }
 }
 
@@ -412,9 +443,16 @@ int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, in
       hipLaunchKernelGGL(k_batch_forces, dim3(grid_particles), dim3(kIbmBlock), 0, s, hdr, dyn,
                          particle_case, particle_markers, lx, lF, lm, pstate);
       if ((rc = check_launch("batch_forces"))) return rc;
+      if (particle_box) {
+        hipLaunchKernelGGL(k_batch_spread_claim, dim3(grid_particles), dim3(kClaimBlock), 0, s, cd, hdr,
+                           dyn, particle_case, particle_markers, case_marker_begin, lx, pstate,
+                           particle_box, origin_a, origin_b, origin_ok, fol_a, fol_b,
+                           (unsigned long long*)owner);
+        if ((rc = check_launch("batch_spread_claim"))) return rc;
+      }
       hipLaunchKernelGGL(k_batch_spread_prep, dim3(grid_markers), dim3(kIbmBlock), 0, s, cd, hdr, dyn,
                          marker_case, case_marker_begin, lx, origin_a, origin_b, origin_ok, fol_a,
-                         fol_b, (unsigned long long*)owner);
+                         fol_b, (unsigned long long*)owner, marker_particle, particle_box);
       if ((rc = check_launch("batch_spread_prep"))) return rc;
       hipLaunchKernelGGL(k_batch_spread_write, dim3(grid_markers), dim3(kIbmBlock), 0, s, cd, hdr,
                          dyn, marker_case, case_marker_begin, lx, lF, lm, origin_a, origin_b,

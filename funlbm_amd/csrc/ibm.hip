@@ -98,18 +98,41 @@ __global__ __launch_bounds__(kIbmBlock) void k_pose(const int* __restrict__ offs
                 advance != 0);
 }
 
+// pid / pbox (may be NULL): markers of particles that k_spread_claim serves
+// are left out.
 __global__ __launch_bounds__(kIbmBlock) void k_spread_prep(KParams p, int nm,
                                                            const float* __restrict__ lx,
                                                            const int* __restrict__ clear_origin,
                                                            float* __restrict__ fol,
                                                            unsigned long long* __restrict__ owner,
-                                                           const int* __restrict__ ctl) {
+                                                           const int* __restrict__ ctl,
+                                                           const int* __restrict__ pid,
+                                                           const int* __restrict__ pbox) {
   __shared__ PrepPeer peers[kIbmBlock / kWave][kMarkersPerWave];
   const int m = blockIdx.x * kMarkersPerBlock + (int)threadIdx.x / kMarkerLanes;
   const int sub = threadIdx.x & (kMarkerLanes - 1);
+  const bool active = m < nm && !(pbox && claims_in_box(pbox + 3 * pid[m]));
   // every lane takes part (spread_prep_marker shares the wave's stencils)
-  spread_prep_marker(p, m < nm, 0, m, m, sub, lx, clear_origin, fol, owner, (uint32_t)(ctl[0] + 1),
+  spread_prep_marker(p, active, 0, m, m, sub, lx, clear_origin, fol, owner, (uint32_t)(ctl[0] + 1),
                      peers[threadIdx.x / kWave]);
+}
+
+// Claims of the particles with a box of at most kClaimNodes nodes, one block
+// per particle (spread_claim_particle).
+__global__ __launch_bounds__(kClaimBlock) void k_spread_claim(KParams p, const int* __restrict__ offsets,
+                                                            const float* __restrict__ lx,
+                                                            const double* __restrict__ ps,
+                                                            const int* __restrict__ pbox,
+                                                            const int* __restrict__ clear_origin,
+                                                            float* __restrict__ fol,
+                                                            unsigned long long* __restrict__ owner,
+                                                            const int* __restrict__ ctl) {
+  __shared__ int own[kClaimNodes];
+  const int i = blockIdx.x;
+  const int* row = pbox + 3 * i;
+  if (!claims_in_box(row)) return;
+  spread_claim_particle(p, offsets[i], offsets[i + 1], offsets[i], lx, pst(ps, i), row, clear_origin,
+                        fol, owner, (uint32_t)(ctl[0] + 1), own);
 }
 
 __global__ __launch_bounds__(kIbmBlock) void k_spread_write(
@@ -270,7 +293,7 @@ int funlbm_ibm_spread(const funlbm_lattice* lat, int32_t n_markers, const float*
   hipStream_t s = (hipStream_t)stream;
   if (n_markers > 0) {
     hipLaunchKernelGGL(k_spread_prep, dim3(wave_grid(n_markers)), dim3(kIbmBlock), 0, s, p, n_markers,
-                       lx, clear_origin, fol, (unsigned long long*)owner, ctl);
+                       lx, clear_origin, fol, (unsigned long long*)owner, ctl, nullptr, nullptr);
     if ((rc = check_launch("ibm_spread"))) return rc;
     hipLaunchKernelGGL(k_spread_write, dim3(wave_grid(n_markers)), dim3(kIbmBlock), 0, s, p, n_markers,
                        lx, lF, lm, fol, (const unsigned long long*)owner, origin, fol_tiles, ctl);
@@ -278,6 +301,40 @@ int funlbm_ibm_spread(const funlbm_lattice* lat, int32_t n_markers, const float*
   }
   hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, s, ctl);
   return check_launch("ibm_spread");
+}
+
+int funlbm_ibm_spread_boxed(const funlbm_lattice* lat, int32_t n_markers, const float* lx,
+                            const float* lF, const float* lm, float* fol, const int32_t* clear_origin,
+                            int32_t* origin, uint64_t* owner, uint32_t* fol_tiles, int32_t* ctl,
+                            int32_t n_particles, const int32_t* marker_offsets,
+                            const int32_t* marker_particle, const double* pstate,
+                            const int32_t* particle_box, void* stream) {
+  KParams p;
+  int rc = make_params(lat, &p);
+  if (rc) return rc;
+  if (n_markers < 0 || n_particles < 0)
+    return set_error(FUNLBM_EINVAL, "ibm_spread_boxed: negative count");
+  if (!fol || !owner || !ctl) return set_error(FUNLBM_EINVAL, "ibm_spread_boxed: NULL array");
+  if (n_markers > 0 && (!lx || !lF || !lm || !marker_offsets || !marker_particle || !pstate ||
+                        !particle_box))
+    return set_error(FUNLBM_EINVAL, "ibm_spread_boxed: NULL array");
+  hipStream_t s = (hipStream_t)stream;
+  if (n_markers > 0) {
+    if (n_particles > 0) {
+      hipLaunchKernelGGL(k_spread_claim, dim3(n_particles), dim3(kClaimBlock), 0, s, p, marker_offsets, lx,
+                         pstate, particle_box, clear_origin, fol, (unsigned long long*)owner, ctl);
+      if ((rc = check_launch("ibm_spread_boxed"))) return rc;
+    }
+    hipLaunchKernelGGL(k_spread_prep, dim3(wave_grid(n_markers)), dim3(kIbmBlock), 0, s, p, n_markers,
+                       lx, clear_origin, fol, (unsigned long long*)owner, ctl, marker_particle,
+                       particle_box);
+    if ((rc = check_launch("ibm_spread_boxed"))) return rc;
+    hipLaunchKernelGGL(k_spread_write, dim3(wave_grid(n_markers)), dim3(kIbmBlock), 0, s, p, n_markers,
+                       lx, lF, lm, fol, (const unsigned long long*)owner, origin, fol_tiles, ctl);
+    if ((rc = check_launch("ibm_spread_boxed"))) return rc;
+  }
+  hipLaunchKernelGGL(k_bump, dim3(1), dim3(1), 0, s, ctl);
+  return check_launch("ibm_spread_boxed");
 }
 
 }  // extern "C"

@@ -655,6 +655,89 @@ __device__ inline void spread_prep_marker(const KParams& p, bool active, int tag
   }
 }
 
+// ---------------------------------------------------------------------------
+// Claims of one particle's markers through its box, in shared memory.  Each
+// marker raises own[node] (the box's nodes, shared memory) to its index in
+// the particle with a shared-memory atomicMax: the particle's owner of every
+// node it covers, a segmented maximum over its markers.  Then each owned node
+// takes one global claim (stamp, key0 + owner); a node two particles cover
+// gets the larger key, as with one claim per marker.  A stencil node outside
+// the box (the centre moved in this step's rigid-body update, a centre that
+// is not finite) is claimed globally at once.  Moved stencils are cleared as
+// in spread_prep_marker.  Particles whose box exceeds kClaimNodes nodes, or
+// that have none, go through spread_prep_marker.  Call with the whole block.
+// ---------------------------------------------------------------------------
+constexpr int kClaimNodes = 16384;  // int32 per node: 64 KiB of shared memory
+constexpr int kClaimBlock = 1024;   // threads of a claiming block: 64 markers per pass
+
+__device__ inline bool claims_in_box(const int* __restrict__ row) {
+  const int R = row[1];
+  if (R < 0 || R > 10) return false;
+  const int S = 2 * R + 4;
+  return S * S * S <= kClaimNodes;
+}
+
+__device__ inline void spread_claim_particle(const KParams& p, int b, int e, int key0,
+                                             const float* __restrict__ lx,
+                                             const double* __restrict__ P,
+                                             const int* __restrict__ row,
+                                             const int* __restrict__ clear_origin,
+                                             float* __restrict__ fol,
+                                             unsigned long long* __restrict__ owner,
+                                             uint32_t stamp, int* own) {
+  const Box box = box_of(P, row, 0x7fffffff);
+  const int S = box.S, n3 = S * S * S;
+  for (int t = threadIdx.x; t < n3; t += blockDim.x) own[t] = -1;
+  __syncthreads();
+  const unsigned long long hi = (unsigned long long)stamp << 32;
+  const int sub = threadIdx.x & (kMarkerLanes - 1);
+  const int j = sub >> 2, k = sub & 3;
+  const int per_pass = (int)blockDim.x / kMarkerLanes;
+  for (int m0 = b; m0 < e; m0 += per_pass) {
+    const int m = m0 + (int)threadIdx.x / kMarkerLanes;
+    if (m >= e) continue;
+    const float x[3] = {lx[3 * m], lx[3 * m + 1], lx[3 * m + 2]};
+    const Stencil s = stencil_of(p, x);
+    if (clear_origin) {
+      const int ol[3] = {clear_origin[3 * m], clear_origin[3 * m + 1], clear_origin[3 * m + 2]};
+      const int oy = ol[1] + j, oz = ol[2] + k;
+      if ((ol[0] != s.lo[0] || ol[1] != s.lo[1] || ol[2] != s.lo[2]) && oy < p.ny && oz < p.nz) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (ol[0] + i >= p.nx) break;
+          const int64_t node = lin(p, ol[0] + i, oy, oz);
+#pragma unroll
+          for (int d = 0; d < 3; ++d) fol[(int64_t)d * p.n + node] = 0.0f;
+        }
+      }
+    }
+    const int ny_ = s.lo[1] + j, nz_ = s.lo[2] + k;
+    if (ny_ >= s.hi[1] || nz_ >= s.hi[2]) continue;
+    const int lj = ny_ - box.o[1], lk = nz_ - box.o[2];
+    const bool in_yz = box.ok && (unsigned)lj < (unsigned)S && (unsigned)lk < (unsigned)S;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int nx_ = s.lo[0] + i;
+      if (nx_ >= s.hi[0]) break;
+      const int li = nx_ - box.o[0];
+      if (in_yz && (unsigned)li < (unsigned)S)
+        atomicMax(&own[(li * S + lj) * S + lk], m - b);
+      else
+        atomicMax(&owner[lin(p, nx_, ny_, nz_)], hi | (unsigned)(key0 + (m - b)));
+    }
+  }
+  __syncthreads();
+  if (!box.ok) return;
+  for (int t = threadIdx.x; t < n3; t += blockDim.x) {
+    const int w = own[t];
+    if (w < 0) continue;
+    const int lk = t % S, r = t / S;
+    const int lj = r % S, li = r / S;
+    atomicMax(&owner[lin(p, box.o[0] + li, box.o[1] + lj, box.o[2] + lk)],
+              hi | (unsigned)(key0 + w));
+  }
+}
+
 __device__ inline void spread_write_marker(const KParams& p, int m, int key_id, int sub,
                                            const float* __restrict__ lx,
                                            const float* __restrict__ lF,

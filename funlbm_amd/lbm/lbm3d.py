@@ -27,9 +27,10 @@ class LBMD3(LBMBase):
     def __init__(self, *args, **kwargs):
         super().__init__(*args, **kwargs)
         # interpolation from the fused state through the particles' moment
-        # boxes (same results as the per-node pull, which False selects);
-        # None: boxes from BOXED_MIN_MARKERS markers up, where they pay for
-        # their two extra launches
+        # boxes, and spreading claims per particle box (same results as the
+        # per-node pull and per-marker claims, which False selects); None:
+        # boxes from BOXED_MIN_MARKERS markers up, where they pay for their
+        # extra launches
         self.boxed_interp = None
 
     def _init(self):
@@ -59,12 +60,14 @@ class LBMD3(LBMBase):
     # ------------------------------------------------------------------
     # IBM phases
     # ------------------------------------------------------------------
+    def _boxed(self) -> bool:
+        if self.boxed_interp is None:
+            return self.particle_swarm.n_markers >= BOXED_MIN_MARKERS
+        return bool(self.boxed_interp)
+
     def _interp(self, from_state: bool):
         sw, fl = self.particle_swarm, self.flow
-        boxed = self.boxed_interp
-        if boxed is None:
-            boxed = sw.n_markers >= BOXED_MIN_MARKERS
-        if from_state and boxed:
+        if from_state and self._boxed():
             state, rface = fl.state_source()
             nch = 0 if self._box_chunks is None else int(self._box_chunks.numel())
             nat.call(
@@ -110,11 +113,14 @@ class LBMD3(LBMBase):
         if fl._fol_live[j] and not self._origin_ok[j]:
             buf.zero_()
         clear = nat.ptr(self._origin[j]) if (fl._fol_live[j] and self._origin_ok[j]) else None
-        nat.call(
-            "funlbm_ibm_spread", fl.lat32, sw.n_markers, nat.ptr(sw.lx), nat.ptr(sw.lF), nat.ptr(sw.lm),
-            nat.ptr(buf), clear, nat.ptr(self._origin[j]), nat.ptr(self._owner), nat.ptr(self._tiles),
-            nat.ptr(sw.ctl), fl._stream(),
-        )
+        args = (fl.lat32, sw.n_markers, nat.ptr(sw.lx), nat.ptr(sw.lF), nat.ptr(sw.lm), nat.ptr(buf), clear,
+                nat.ptr(self._origin[j]), nat.ptr(self._owner), nat.ptr(self._tiles), nat.ptr(sw.ctl))
+        if self._boxed():
+            # claims per particle box in shared memory (same ownership)
+            nat.call("funlbm_ibm_spread_boxed", *args, len(sw.particles), nat.ptr(sw.offsets),
+                     nat.ptr(sw.pid), nat.ptr(sw.pstate), nat.ptr(self._box_rows), fl._stream())
+        else:
+            nat.call("funlbm_ibm_spread", *args, fl._stream())
         self._origin_ok[j] = True
 
     def lagrange_to_flow(self, *args, **kwargs):

@@ -297,6 +297,20 @@ int funlbm_ibm_spread(const funlbm_lattice* lat, int32_t n_markers, const float*
                       const float* lm, float* fol, const int32_t* clear_origin, int32_t* origin,
                       uint64_t* owner, uint32_t* fol_tiles, int32_t* ctl, void* stream);
 
+/* funlbm_ibm_spread with the claims of each particle whose box (the rows of
+ * funlbm_ibm_interp_boxed; a box of at most 16384 nodes) holds its stencils
+ * formed in shared memory: a segmented maximum of the marker index per node
+ * of the box, then one global claim per node and particle.  The other
+ * particles' markers claim per node as in funlbm_ibm_spread.  Same results
+ * (the overwrite order of src/funlbm/lbm/lbm3d.py:102-119).
+ * marker_offsets[n_particles + 1] bound each particle's markers. */
+int funlbm_ibm_spread_boxed(const funlbm_lattice* lat, int32_t n_markers, const float* lx,
+                            const float* lF, const float* lm, float* fol, const int32_t* clear_origin,
+                            int32_t* origin, uint64_t* owner, uint32_t* fol_tiles, int32_t* ctl,
+                            int32_t n_particles, const int32_t* marker_offsets,
+                            const int32_t* marker_particle, const double* pstate,
+                            const int32_t* particle_box, void* stream);
+
 /* ---- many lattices per launch ----
  *
  * Independent cases of uneven size share pools: two state pools (the case's

@@ -214,10 +214,12 @@ BOX_PARTS = [
 @pytest.mark.parametrize("q", [19, 27])
 def test_boxed_interpolation_equals_per_node_pull(q, storage):
     """The interpolation through the particles' moment boxes gives the same
-    bits as pulling every stencil node per marker: marker forces, particle
-    states and populations, over steps whose markers start at the origin
-    (step 1: every lane outside the boxes), then lie in them, with stencils
-    clamped at a wall and boxes that overlap."""
+    bits as pulling every stencil node per marker, and the spreading claims
+    formed per particle box the same ownership as per-marker claims: marker
+    forces, particle states, FOL and populations, over steps whose markers
+    start at the origin (step 1: every lane outside the boxes), then lie in
+    them, with stencils clamped at a wall and boxes that overlap (two
+    particles claim shared nodes)."""
     size = (27, 23, 23)
     bcs = (W, W, P, P, P, P)
     runs = []
@@ -238,6 +240,7 @@ def test_boxed_interpolation_equals_per_node_pull(q, storage):
     assert torch.equal(sa.lF, sb.lF)
     assert torch.equal(sa.lu, sb.lu)
     assert torch.equal(sa.pstate, sb.pstate)
+    assert torch.equal(a.flow.FOL, b.flow.FOL)
     assert torch.equal(a.flow.f, b.flow.f)  # the state's nodes (its slack tiles hold no data)
     # the last box pass wrote every node of each box inside the lattice (the
     # cache started as NaN; the centre moved less than a node since then)
