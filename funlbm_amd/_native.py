@@ -107,7 +107,8 @@ SIGNATURES = {
     "funlbm_particle_forces": (c_int32, [c_int32, P, P, P, P, P, ctypes.c_double, ctypes.c_double, P]),
     "funlbm_particle_pose": (c_int32, [c_int32, P, P, P, P, c_int32, P]),
     "funlbm_ibm_spread": (c_int32, [LATP, c_int32, P, P, P, P, P, P, P, P, P, P]),
-    "funlbm_ibm_spread_boxed": (c_int32, [LATP, c_int32, P, P, P, P, P, P, P, P, P, c_int32, P, P, P, P, P]),
+    "funlbm_ibm_spread_boxed": (c_int32, [LATP, c_int32, P, P, P, P, P, P, P, P, P, c_int32, P, P, P, P, c_int32,
+                                          P]),
     "funlbm_surface_markers": (c_int32, [c_int32, P, ctypes.c_double, c_int32, P, P, P]),
     "funlbm_flow_stats": (c_int32, [LATP, P, P, P, P, P, P, P, c_int32, P, P]),
     "funlbm_case_bytes": (c_int32, []),

@@ -213,7 +213,7 @@ class BatchEngine:
         self._graph = None
         self._graph_steps = 0
         self._graph_mask = 0
-        self._mask = 0  # lattices with cases (bit i: D3Q13, 15, 19, 27)
+        self._mask = 0  # lattices with cases (bit i: D3Q13, 15, 19, 27), bit 4: per-marker claims
         self._dirty = True
 
     # ------------------------------------------------------------------ admission
@@ -354,6 +354,9 @@ class BatchEngine:
         nt = ends[-1]
         self._mask = sum(1 << i for i, q in enumerate((13, 15, 19, 27))
                          if any(s.spec.q == q for s in self.slots.values()))
+        # per-marker spreading claims while a particle has no shared-memory box
+        if any(not boxes.claims_in_box(R) for s in self.slots.values() for _, R in s.boxes):
+            self._mask |= 16
         if nt:
             self.tile_case[:nt] = torch.from_numpy(np.concatenate(tc)).to(self.device)
             self.tile_local[:nt] = torch.from_numpy(np.concatenate(tl)).to(self.device)

@@ -450,9 +450,11 @@ int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, in
                            (unsigned long long*)owner);
         if ((rc = check_launch("batch_spread_claim"))) return rc;
       }
-      hipLaunchKernelGGL(k_batch_spread_prep, dim3(grid_markers), dim3(kIbmBlock), 0, s, cd, hdr, dyn,
-                         marker_case, case_marker_begin, lx, origin_a, origin_b, origin_ok, fol_a,
-                         fol_b, (unsigned long long*)owner, marker_particle, particle_box);
+      // per-marker claims only where some particle has no shared-memory box
+      if (!particle_box || (lattice_mask & 16))
+        hipLaunchKernelGGL(k_batch_spread_prep, dim3(grid_markers), dim3(kIbmBlock), 0, s, cd, hdr,
+                           dyn, marker_case, case_marker_begin, lx, origin_a, origin_b, origin_ok,
+                           fol_a, fol_b, (unsigned long long*)owner, marker_particle, particle_box);
       if ((rc = check_launch("batch_spread_prep"))) return rc;
       hipLaunchKernelGGL(k_batch_spread_write, dim3(grid_markers), dim3(kIbmBlock), 0, s, cd, hdr,
                          dyn, marker_case, case_marker_begin, lx, lF, lm, origin_a, origin_b,

@@ -308,7 +308,8 @@ int funlbm_ibm_spread_boxed(const funlbm_lattice* lat, int32_t n_markers, const 
                             int32_t* origin, uint64_t* owner, uint32_t* fol_tiles, int32_t* ctl,
                             int32_t n_particles, const int32_t* marker_offsets,
                             const int32_t* marker_particle, const double* pstate,
-                            const int32_t* particle_box, void* stream) {
+                            const int32_t* particle_box, int32_t claims_per_marker,
+                            void* stream) {
   KParams p;
   int rc = make_params(lat, &p);
   if (rc) return rc;
@@ -325,10 +326,12 @@ int funlbm_ibm_spread_boxed(const funlbm_lattice* lat, int32_t n_markers, const 
                          pstate, particle_box, clear_origin, fol, (unsigned long long*)owner, ctl);
       if ((rc = check_launch("ibm_spread_boxed"))) return rc;
     }
-    hipLaunchKernelGGL(k_spread_prep, dim3(wave_grid(n_markers)), dim3(kIbmBlock), 0, s, p, n_markers,
-                       lx, clear_origin, fol, (unsigned long long*)owner, ctl, marker_particle,
-                       particle_box);
-    if ((rc = check_launch("ibm_spread_boxed"))) return rc;
+    if (claims_per_marker) {
+      hipLaunchKernelGGL(k_spread_prep, dim3(wave_grid(n_markers)), dim3(kIbmBlock), 0, s, p, n_markers,
+                         lx, clear_origin, fol, (unsigned long long*)owner, ctl, marker_particle,
+                         particle_box);
+      if ((rc = check_launch("ibm_spread_boxed"))) return rc;
+    }
     hipLaunchKernelGGL(k_spread_write, dim3(wave_grid(n_markers)), dim3(kIbmBlock), 0, s, p, n_markers,
                        lx, lF, lm, fol, (const unsigned long long*)owner, origin, fol_tiles, ctl);
     if ((rc = check_launch("ibm_spread_boxed"))) return rc;

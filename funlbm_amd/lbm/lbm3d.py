@@ -56,6 +56,7 @@ class LBMD3(LBMBase):
             self._box_cache = torch.empty(4 * max(nodes, 1), device=dev, dtype=torch.float32)
             self._box_nodes = nodes
             self._box_miss = torch.zeros(sw.n_markers + 1, device=dev, dtype=torch.int32)
+            self._claims_per_marker = 0 if all(boxes.claims_in_box(R) for R in radii) else 1
 
     # ------------------------------------------------------------------
     # IBM phases
@@ -118,7 +119,8 @@ class LBMD3(LBMBase):
         if self._boxed():
             # claims per particle box in shared memory (same ownership)
             nat.call("funlbm_ibm_spread_boxed", *args, len(sw.particles), nat.ptr(sw.offsets),
-                     nat.ptr(sw.pid), nat.ptr(sw.pstate), nat.ptr(self._box_rows), fl._stream())
+                     nat.ptr(sw.pid), nat.ptr(sw.pstate), nat.ptr(self._box_rows),
+                     self._claims_per_marker, fl._stream())
         else:
             nat.call("funlbm_ibm_spread", *args, fl._stream())
         self._origin_ok[j] = True

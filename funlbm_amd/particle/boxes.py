@@ -39,6 +39,16 @@ def box_radius(lag: np.ndarray) -> int:
     return R
 
 
+# boxes whose claims a block forms in shared memory (at most 16384 nodes)
+CLAIM_MAX_R = 10
+
+
+def claims_in_box(R: int) -> bool:
+    """Whether a particle of box radius R claims its spreading nodes through
+    its box in shared memory (otherwise per marker)."""
+    return 0 <= R <= CLAIM_MAX_R
+
+
 def box_nodes(R: int) -> int:
     """Nodes of the cube of a box of radius R (0 for no box)."""
     return (2 * R + 4) ** 3 if R >= 0 else 0

@@ -303,13 +303,17 @@ int funlbm_ibm_spread(const funlbm_lattice* lat, int32_t n_markers, const float*
  * of the box, then one global claim per node and particle.  The other
  * particles' markers claim per node as in funlbm_ibm_spread.  Same results
  * (the overwrite order of src/funlbm/lbm/lbm3d.py:102-119).
- * marker_offsets[n_particles + 1] bound each particle's markers. */
+ * marker_offsets[n_particles + 1] bound each particle's markers.
+ * claims_per_marker = 0 declares that every particle has such a box
+ * (0 <= R <= 10) and skips the per-marker claim kernel; a particle without
+ * one then claims nothing. */
 int funlbm_ibm_spread_boxed(const funlbm_lattice* lat, int32_t n_markers, const float* lx,
                             const float* lF, const float* lm, float* fol, const int32_t* clear_origin,
                             int32_t* origin, uint64_t* owner, uint32_t* fol_tiles, int32_t* ctl,
                             int32_t n_particles, const int32_t* marker_offsets,
                             const int32_t* marker_particle, const double* pstate,
-                            const int32_t* particle_box, void* stream);
+                            const int32_t* particle_box, int32_t claims_per_marker,
+                            void* stream);
 
 /* ---- many lattices per launch ----
  *
@@ -343,7 +347,9 @@ int funlbm_case_pack(const funlbm_lattice* lat, int64_t f_offset, int64_t node_o
  * miss_list (int32, marker slots + 1) are the moment boxes of
  * funlbm_ibm_interp_boxed, over particle slots.  lattice_mask (bits 0-3:
  * D3Q13, D3Q15, D3Q19, D3Q27) selects the fused kernels launched; a lattice
- * whose bit is clear is not stepped.  The grid sizes fix the persistent
+ * whose bit is clear is not stepped.  Bit 4 runs the per-marker spreading
+ * claims, needed while some particle has no box of 0 <= R <= 10 (the
+ * others claim through their boxes, as in funlbm_ibm_spread_boxed).  The grid sizes fix the persistent
  * launch shapes. */
 int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, int32_t* case_par,
                       int32_t* case_left, int32_t* case_done, int32_t* case_flag,
