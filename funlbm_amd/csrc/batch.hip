@@ -189,32 +189,21 @@ __global__ __launch_bounds__(kBoxChunk) void k_batch_box_moments(
 }
 
 // particle_markers holds [begin, end) marker ranges, two int32 per particle.
-__global__ __launch_bounds__(kIbmBlock) void k_batch_wall(const CaseDesc* __restrict__ cases,
-                                                          const int* __restrict__ hdr, BatchDyn dyn,
-                                                          const int* __restrict__ particle_case,
-                                                          const int* __restrict__ pm,
-                                                          const float* __restrict__ lx,
-                                                          float* __restrict__ lF, float dx) {
+// Wall spring, then force/torque reduction and rigid-body update, of each
+// active particle (one block per particle; the spring's lF rows are read by
+// the reduction after the block barrier).
+__global__ __launch_bounds__(kIbmBlock) void k_batch_wall_forces(
+    const CaseDesc* __restrict__ cases, const int* __restrict__ hdr, BatchDyn dyn,
+    const int* __restrict__ particle_case, const int* __restrict__ pm, const float* __restrict__ lx,
+    float* __restrict__ lF, const float* __restrict__ lm, double* __restrict__ ps, float dx) {
   const int np = hdr[H_PARTICLES];
   for (int i = blockIdx.x; i < np; i += gridDim.x) {
     const int c = particle_case[i];
     if (c < 0 || dyn.left[c] <= 0) continue;
     wall_particle(cases[c].p, pm[2 * i], pm[2 * i + 1], lx, lF, dx);
-  }
-}
-
-__global__ __launch_bounds__(kIbmBlock) void k_batch_forces(const int* __restrict__ hdr, BatchDyn dyn,
-                                                            const int* __restrict__ particle_case,
-                                                            const int* __restrict__ pm,
-                                                            const float* __restrict__ lx,
-                                                            const float* __restrict__ lF,
-                                                            const float* __restrict__ lm,
-                                                            double* __restrict__ ps) {
-  const int np = hdr[H_PARTICLES];
-  for (int i = blockIdx.x; i < np; i += gridDim.x) {
-    const int c = particle_case[i];
-    if (c < 0 || dyn.left[c] <= 0) continue;
+    __syncthreads();
     forces_particle(pm[2 * i], pm[2 * i + 1], lx, lF, lm, pst(ps, i), 1.0, 1.0);
+    __syncthreads();  // shared memory is reused by the next particle
   }
 }
 
@@ -437,12 +426,9 @@ int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, in
                          (const E*)pool_b, fol_a, fol_b, lx, pstate, lu, lF,
                          cache ? (const int*)miss_list : nullptr);
       if ((rc = check_launch("batch_interp_pull"))) return rc;
-      hipLaunchKernelGGL(k_batch_wall, dim3(grid_particles), dim3(kIbmBlock), 0, s, cd, hdr, dyn,
-                         particle_case, particle_markers, lx, lF, 1.0f);
-      if ((rc = check_launch("batch_wall"))) return rc;
-      hipLaunchKernelGGL(k_batch_forces, dim3(grid_particles), dim3(kIbmBlock), 0, s, hdr, dyn,
-                         particle_case, particle_markers, lx, lF, lm, pstate);
-      if ((rc = check_launch("batch_forces"))) return rc;
+      hipLaunchKernelGGL(k_batch_wall_forces, dim3(grid_particles), dim3(kIbmBlock), 0, s, cd, hdr, dyn,
+                         particle_case, particle_markers, lx, lF, lm, pstate, 1.0f);
+      if ((rc = check_launch("batch_wall_forces"))) return rc;
       if (particle_box) {
         hipLaunchKernelGGL(k_batch_spread_claim, dim3(grid_particles), dim3(kClaimBlock), 0, s, cd, hdr,
                            dyn, particle_case, particle_markers, case_marker_begin, lx, pstate,
