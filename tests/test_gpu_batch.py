@@ -170,6 +170,39 @@ def test_graph_follows_a_lattice_admitted_after_capture():
         assert torch.equal(ua, ub) and torch.equal(ra, rb)
 
 
+def test_graph_follows_a_particle_without_shared_memory_box():
+    """A graph captured while every particle claims through its box is
+    captured again when a case arrives whose particle has no box (the
+    per-marker claim kernel is then launched), and every case steps as it
+    would eagerly."""
+    specs = _mixed_specs()
+    first, late = specs[:1], specs[1:3]
+    # a cache that holds the first case's box only
+    kw = dict(storage="f32", max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8,
+              max_box_nodes=3000)
+    eager, graph = BatchEngine(**kw), BatchEngine(**kw)
+    ie = [eager.add(s) for s in first]
+    ig = [graph.add(s) for s in first]
+    eager.step(4)
+    graph.capture(4)
+    graph.run_graph()
+    assert not graph._graph_mask & 16
+    ie += [eager.add(s) for s in late]
+    ig += [graph.add(s) for s in late]
+    assert any(R < 0 for s in graph.slots.values() for _, R in s.boxes)
+    eager.step(8)
+    graph.run_graph()
+    graph.run_graph()
+    torch.cuda.synchronize()
+    assert graph._graph_mask & 16
+    assert graph.case_done.cpu().numpy()[ig].tolist() == eager.case_done.cpu().numpy()[ie].tolist()
+    for a, b in zip(ie, ig):
+        ua, ra = eager.fields(a)
+        ub, rb = graph.fields(b)
+        assert torch.equal(ua, ub) and torch.equal(ra, rb)
+        assert torch.equal(eager.particle_rows(a), graph.particle_rows(b))
+
+
 def test_retire_and_reuse_slot():
     specs = _mixed_specs()
     eng = BatchEngine(storage="f16", max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8)
