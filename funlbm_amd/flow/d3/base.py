@@ -465,6 +465,27 @@ class FlowD3(Flow):
             vals += [t.min().double(), t.double().mean(), t.max().double()]
         return torch.stack(vals)
 
+    def conservation(self):
+        """Global mass and momentum as a double[4] device tensor: sum of rho,
+        then sum of rho u_d (the reference's rho and u, reduced in double).
+        After fused steps it is one read of the state
+        (funlbm_flow_conservation)."""
+        if self._mode == "fused" and not self._mat_valid:
+            if getattr(self, "_cons_part", None) is None:
+                self._cons_part = torch.empty(4 * 2048, device=self.device, dtype=torch.float64)
+                self._cons_out = torch.empty(4, device=self.device, dtype=torch.float64)
+            a = 1 - self._cur
+            nat.call(
+                "funlbm_flow_conservation", self.lat, nat.ptr(self._state[a]), self._fol_ptr(1 - self._fc),
+                self._fol_ptr(self._fc), nat.ptr(self._tiles[0]), nat.ptr(self._tiles[1]),
+                nat.ptr(self._rho_face[a]), nat.ptr(self._cons_part), 2048, nat.ptr(self._cons_out),
+                self._stream(),
+            )
+            return self._cons_out
+        rho = self.rou.double().reshape(-1)
+        u = self.u.double().reshape(-1, 3)
+        return torch.stack([rho.sum()] + [(rho * u[:, d]).sum() for d in range(3)])
+
     def _ensure_stat_out(self):
         if getattr(self, "_stat_out", None) is None:
             self._stat_out = torch.empty(9, device=self.device, dtype=torch.float64)

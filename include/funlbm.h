@@ -378,6 +378,17 @@ int funlbm_flow_stats(const funlbm_lattice* lat, const void* state_src, const fl
                       const float* rho_prev, double* partials, int32_t n_partials, double* out,
                       void* stream);
 
+/* Global mass and momentum after the last fused step, in double: out[0] =
+ * sum of rho, out[1..3] = sum of rho u_d over the nodes (rho and u as the
+ * reference's float32 tensors hold them, the forcing correction included).
+ * Arguments as funlbm_flow_stats; partials: 4 * n_partials doubles.
+ * Deterministic (fixed-order reductions).  No reference counterpart: a
+ * conservation check of the fused state. */
+int funlbm_flow_conservation(const funlbm_lattice* lat, const void* state_src, const float* fol_old,
+                             const float* fol_new, const uint32_t* fol_tiles, const int32_t* ctl,
+                             const float* rho_prev, double* partials, int32_t n_partials, double* out,
+                             void* stream);
+
 /* Plain device copy of n_bytes (16-byte aligned), used to calibrate the
  * measured copy bandwidth.  No reference counterpart. */
 int funlbm_copy_calibrate(const void* src, void* dst, int64_t n_bytes, void* stream);
