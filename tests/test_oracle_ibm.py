@@ -156,3 +156,152 @@ def test_spreading_overwrites_in_marker_order():
     w1 = ibo.weight_function(ibo._grid(rl[1], rr[1]).astype(np.float32) - p.lx[1])
     sl = tuple(slice(rl[1, d], rr[1, d]) for d in range(3))
     assert np.array_equal(st.FOL[sl][..., 0:1], w1 * np.float32(2.0) * p.lm[1])
+
+
+# ---------------------------------------------------------------- interpolation pins
+# flow_to_lagrange (src/funlbm/lbm/lbm3d.py:66-100) against closed forms that
+# do not restate its formula: a uniform field interpolates to itself under
+# any normalised kernel, so s_u is the component sum a + b + c (F7) and
+# s_rho the density; lu is the rigid-body velocity; and on a node the
+# cosine kernel's per-axis weights are (1/4, 1/2, 1/4, 0).
+UNIFORM_U = (0.01, -0.02, 0.005)
+UNIFORM_RHO = 1.07
+RIGID = dict(cu=(0.003, -0.001, 0.002), cw=(0.001, 0.002, -0.0015), cx=(6.2, 7.1, 5.9))
+MARKERS = np.array([[6.7, 7.3, 8.2], [4.1, 9.6, 5.5], [8.05, 6.45, 3.9], [5.0, 6.0, 7.0]], np.float32)
+SIZE = [14, 15, 13]
+DT = 0.5
+
+
+def _uniform_case():
+    nodes = tuple(s + 1 for s in SIZE)
+    st = orc.init_flow(19, nodes, mu=0.05, bc=[orc.PERIODICAL] * 6)
+    st.u = np.broadcast_to(np.array(UNIFORM_U, np.float32), nodes + (3,)).copy()
+    st.rou = np.full(nodes + (1,), UNIFORM_RHO, np.float32)
+    p = ibo.Particle(kind="sphere", r=3, lagrange=np.zeros((len(MARKERS), 3), np.float32))
+    p.lx = MARKERS.copy()
+    p.cu = np.array(RIGID["cu"])
+    p.cw = np.array(RIGID["cw"])
+    p.cx = np.array(RIGID["cx"])
+    return st, p
+
+
+def _closed_form_lu_lF():
+    cu, cw, cx = (np.array(RIGID[k]) for k in ("cu", "cw", "cx"))
+    lu = cu + np.cross(cw, MARKERS.astype(np.float64) - cx)
+    s_u = sum(UNIFORM_U)
+    lF = 2.0 * UNIFORM_RHO * (lu - s_u) / DT
+    return lu, lF
+
+
+def _interp_pins_hold(f2l):
+    """True if flow_to_lagrange `f2l` meets the closed forms above."""
+    st, p = _uniform_case()
+    f2l(st, [p], SIZE, DT, 1)
+    lu, lF = _closed_form_lu_lF()
+    scale = np.max(np.abs(lF))
+    return bool(np.allclose(p.lu, lu, rtol=0, atol=1e-7) and np.max(np.abs(p.lF - lF)) < 1e-5 * scale)
+
+
+def test_interpolation_of_uniform_field_closed_form():
+    """s_u = a + b + c (the components summed into one scalar, F7),
+    s_rho = rho, lu = cu + cw x (lx - cx), lF = 2 rho (lu - s_u)/dt."""
+    assert _interp_pins_hold(ibo.flow_to_lagrange)
+    st, p = _uniform_case()
+    ibo.flow_to_lagrange(st, [p], SIZE, DT, 1)
+    lu, lF = _closed_form_lu_lF()
+    # every component of lF carries the same scalar s_u
+    s_u = lu - p.lF.astype(np.float64) * DT / (2.0 * UNIFORM_RHO)
+    assert np.allclose(s_u, sum(UNIFORM_U), rtol=0, atol=2e-7)
+
+
+def test_cosine_weights_on_a_node_and_clamped_at_a_face():
+    """A marker on a node: per-axis weights (1/4, 1/2, 1/4, 0) on the stencil
+    [x-1, x+3), so the 3-D weights are their products and need no
+    normalisation.  A marker at x = 0.3 (stencil clamped to [0, 4)): the
+    weights phi(0.3), phi(0.7), phi(1.7) and 0 (|r| = 2.7 > 2) along x, divided
+    by their sum (the node at x = -1 is missing)."""
+    x = np.array([[5.0, 6.0, 7.0]], np.float32)
+    rl, rr = ibo.region_bounds(x, SIZE)
+    assert rl[0].tolist() == [4, 5, 6] and rr[0].tolist() == [8, 9, 10]
+    w = ibo.weight_function(ibo._grid(rl[0], rr[0]).astype(np.float32) - x[0])[..., 0]
+    axis = np.array([0.25, 0.5, 0.25, 0.0])
+    want = axis[:, None, None] * axis[None, :, None] * axis[None, None, :]
+    assert np.allclose(w, want, rtol=0, atol=1e-7)
+
+    x = np.array([[0.3, 6.0, 7.0]], np.float32)
+    rl, rr = ibo.region_bounds(x, SIZE)
+    assert rl[0].tolist() == [0, 5, 6] and rr[0].tolist() == [4, 9, 10]
+    w = ibo.weight_function(ibo._grid(rl[0], rr[0]).astype(np.float32) - x[0])[..., 0]
+    phi = [(1 + math.cos(math.pi * r / 2)) / 4 for r in (0.3, 0.7, 1.7)]
+    ax = np.array(phi + [0.0]) / sum(phi)
+    want = ax[:, None, None] * axis[None, :, None] * axis[None, None, :]
+    assert np.allclose(w, want, rtol=0, atol=2e-7)
+    assert abs(float(w.sum()) - 1.0) < 1e-6
+
+
+def _mutant_f2l(per_component=False, factor=2.0, sign=1.0, kernel_scale=0.5):
+    """flow_to_lagrange with one planted mistake: components interpolated
+    separately, another prefactor, the sign of (lu - s_u) flipped, or the
+    kernel's argument pi r scaled differently."""
+
+    def weights(r):
+        mask = np.all(np.abs(r) <= 2, axis=-1, keepdims=True)
+        tmp = np.where(mask, (np.float32(1) + np.cos(np.abs(r * np.float32(np.pi) * np.float32(kernel_scale))))
+                       / np.float32(4), np.float32(0)).astype(np.float32)
+        w = (tmp[..., 0] * tmp[..., 1] * tmp[..., 2])[..., None]
+        s = np.sum(w)
+        return (w / s if s > 0 else w).astype(np.float32)
+
+    def f2l(st, particles, size, dt, step):
+        for p in particles:
+            rl, rr = ibo.region_bounds(p.lx, size)
+            lu = np.zeros_like(p.lu)
+            lrou = np.zeros((len(p.lx), 1), np.float32)
+            for m, lar in enumerate(p.lx):
+                sl = tuple(slice(rl[m, d], rr[m, d]) for d in range(3))
+                w = weights(ibo._grid(rl[m], rr[m]).astype(np.float32) - lar)
+                lu[m, :] = np.sum(st.u[sl] * w, axis=(0, 1, 2)) if per_component else np.sum(st.u[sl] * w)
+                lrou[m, :] = np.sum(st.rou[sl] * w)
+            p.lu = (p.cu + np.cross(p.cw, p.lx.astype(np.float64) - p.cx)).astype(np.float32)
+            p.lF = (np.float32(factor) * lrou * np.float32(sign) * (p.lu - lu) / np.float32(dt)).astype(np.float32)
+        return True
+
+    return f2l
+
+
+@pytest.mark.parametrize("mutation", [dict(per_component=True), dict(factor=1.0), dict(sign=-1.0),
+                                      dict(kernel_scale=1.0), dict(kernel_scale=0.25)])
+def test_interpolation_pins_catch_planted_mistakes(mutation):
+    """Each planted mistake fails a pin above: the uniform field catches the
+    per-component sum, the factor and the sign; the on-node weights catch a
+    wrong kernel argument (the mutant itself is normalised, so the uniform
+    field alone cannot)."""
+    assert _interp_pins_hold(_mutant_f2l())  # the unmutated copy passes
+    f2l = _mutant_f2l(**mutation)
+    caught = not _interp_pins_hold(f2l)
+    if "kernel_scale" in mutation:
+        # the on-node pin, through the mutant's weights on a marker at a node
+        st, p = _uniform_case()
+        st.u = np.zeros_like(st.u)
+        st.u[..., 0] = np.arange(st.u.shape[0], dtype=np.float32)[:, None, None]  # u_x = x
+        p.lx = np.array([[5.0, 6.0, 7.0]], np.float32)
+        p.cu = np.zeros(3)
+        p.cw = np.zeros(3)
+        p.lu = np.zeros((1, 3), np.float32)
+        f2l(st, [p], SIZE, 1.0, 1)
+        # the (1/4, 1/2, 1/4, 0) weights reproduce u_x = 5 on the node
+        caught = caught or abs(float(p.lF[0, 0]) / (-2.0 * UNIFORM_RHO) - 5.0) > 1e-5
+    assert caught, mutation
+
+
+def test_on_node_interpolation_of_a_linear_field():
+    """u_x = x sampled at a node: the weights (1/4, 1/2, 1/4, 0) average
+    x - 1, x, x + 1 to x exactly (the oracle's result, to float32)."""
+    st, p = _uniform_case()
+    st.u = np.zeros_like(st.u)
+    st.u[..., 0] = np.arange(st.u.shape[0], dtype=np.float32)[:, None, None]
+    p.lx = np.array([[5.0, 6.0, 7.0]], np.float32)
+    p.cu = np.zeros(3)
+    p.cw = np.zeros(3)
+    ibo.flow_to_lagrange(st, [p], SIZE, 1.0, 1)
+    assert abs(float(p.lF[0, 0]) - (-2.0 * UNIFORM_RHO * 5.0)) < 1e-5
