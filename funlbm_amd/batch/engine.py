@@ -216,6 +216,14 @@ class BatchEngine:
         self._mask = 0  # lattices with cases (bit i: D3Q13, 15, 19, 27), bit 4: per-marker claims
         self._dirty = True
 
+    def _h2d(self, arr):
+        """A host array on the device, copied asynchronously on the current
+        stream through pinned memory: admission and publishing then do not
+        wait for steps already in flight (the pinned block is not reused
+        before its copy has run)."""
+        t = torch.from_numpy(np.ascontiguousarray(arr)).pin_memory()
+        return t.to(self.device, non_blocking=True)
+
     # ------------------------------------------------------------------ admission
     def fits(self, spec: CaseSpec) -> bool:
         m = sum(len(markers_for(p)[0]) for p in spec.particles)
@@ -263,9 +271,8 @@ class BatchEngine:
         lat = lattice_for(spec, self.storage, self.dev_scale, self.rounding)
         desc = (ctypes.c_uint8 * self.case_bytes)()
         nat.call("funlbm_case_pack", lat, f_off, node_off, tile_off, desc)
-        self.cases[cid * self.case_bytes : (cid + 1) * self.case_bytes] = torch.frombuffer(
-            bytearray(desc), dtype=torch.uint8
-        ).to(dev)
+        self.cases[cid * self.case_bytes : (cid + 1) * self.case_bytes] = self._h2d(
+            np.frombuffer(bytes(desc), dtype=np.uint8))
         # state: the reference's init (f = w, its moments, the two collisions
         # before the first stream), as LBMD3 enters its fused steps
         esz = self.pool[0].element_size()
@@ -273,7 +280,10 @@ class BatchEngine:
                  nat.stream_handle(dev))
         for j in range(2):
             self.fol[j][3 * node_off : 3 * (node_off + n)].zero_()
+        # the case's spreading stamps restart at 1: clear the tile entries and
+        # owner words an earlier case left in these ranges
         self.fol_tiles[tile_off : tile_off + n_tiles].zero_()
+        self.owner[node_off : node_off + n].zero_()
         # markers and particles
         ps = np.zeros((len(spec.particles), nat.PSTRIDE))
         pm = np.zeros(2 * len(spec.particles), dtype=np.int32)
@@ -293,21 +303,20 @@ class BatchEngine:
             ps[k, nat.P_GRAV : nat.P_GRAV + 3] = np.array(spec.gravity, np.float32)
         if m_count:
             sl = slice(3 * m_off, 3 * (m_off + m_count))
-            self.lag[sl] = torch.from_numpy(np.concatenate(lags).reshape(-1)).to(dev)
+            self.lag[sl] = self._h2d(np.concatenate(lags).reshape(-1))
             self.lx[sl].zero_()
             self.lu[sl].zero_()
             self.lF[sl].zero_()
-            self.lm[m_off : m_off + m_count] = torch.from_numpy(np.concatenate(lms)).to(dev)
-            self.marker_particle[m_off : m_off + m_count] = torch.from_numpy(np.concatenate(mp)).to(dev)
+            self.lm[m_off : m_off + m_count] = self._h2d(np.concatenate(lms))
+            self.marker_particle[m_off : m_off + m_count] = self._h2d(np.concatenate(mp))
             self.marker_case[m_off : m_off + m_count] = cid
         if spec.particles:
             P = len(spec.particles)
-            self.pstate[p_off * nat.PSTRIDE : (p_off + P) * nat.PSTRIDE] = torch.from_numpy(
-                ps.reshape(-1)).to(dev)
-            self.particle_markers[2 * p_off : 2 * (p_off + P)] = torch.from_numpy(pm).to(dev)
+            self.pstate[p_off * nat.PSTRIDE : (p_off + P) * nat.PSTRIDE] = self._h2d(ps.reshape(-1))
+            self.particle_markers[2 * p_off : 2 * (p_off + P)] = self._h2d(pm)
             self.particle_case[p_off : p_off + P] = cid
             self.traj[p_off * self.traj_cap * 4 : (p_off + P) * self.traj_cap * 4].zero_()
-        dyn = torch.tensor([0, int(spec.steps), 0, 0, 0, m_off], dtype=torch.int32).to(dev)
+        dyn = self._h2d(np.array([0, int(spec.steps), 0, 0, 0, m_off], dtype=np.int32))
         self.case_par[cid] = dyn[0]
         self.case_left[cid] = dyn[1]
         self.case_done[cid] = dyn[2]
@@ -358,8 +367,8 @@ class BatchEngine:
         if any(not boxes.claims_in_box(R) for s in self.slots.values() for _, R in s.boxes):
             self._mask |= 16
         if nt:
-            self.tile_case[:nt] = torch.from_numpy(np.concatenate(tc)).to(self.device)
-            self.tile_local[:nt] = torch.from_numpy(np.concatenate(tl)).to(self.device)
+            self.tile_case[:nt] = self._h2d(np.concatenate(tc))
+            self.tile_local[:nt] = self._h2d(np.concatenate(tl))
         # moment boxes over particle slots, and the chunk list of the box kernel
         radii = np.full(self.max_particles, -1, dtype=np.int64)
         offs = np.zeros(self.max_particles, dtype=np.int64)
@@ -367,13 +376,14 @@ class BatchEngine:
             for k, (off, R) in enumerate(s.boxes):
                 radii[s.p_off + k], offs[s.p_off + k] = R, off
         rows, chunks, _ = boxes.layout(radii, offs)
-        self.particle_box.copy_(torch.from_numpy(rows.reshape(-1)))
+        self.particle_box.copy_(self._h2d(rows.reshape(-1).astype(np.int32)))
         nch = len(chunks)
         if nch:
-            self.chunk_particle[:nch] = torch.from_numpy(chunks).to(self.device)
-        stamp = int(self.hdr[H_STAMP].item())
-        hdr = [nt, self.markers.high_water, self.parts.high_water, self.max_cases, stamp] + ends[:3] + [nch]
-        self.hdr.copy_(torch.tensor(hdr, dtype=torch.int32))
+            self.chunk_particle[:nch] = self._h2d(chunks.astype(np.int32))
+        # hdr[H_STAMP] (steps launched) stays as the device has it
+        head = [nt, self.markers.high_water, self.parts.high_water, self.max_cases]
+        self.hdr[:H_STAMP] = self._h2d(np.array(head, dtype=np.int32))
+        self.hdr[H_STAMP + 1 :] = self._h2d(np.array(ends[:3] + [nch], dtype=np.int32))
         self._with_ibm = 1 if self.markers.high_water > 0 else 0
         self._dirty = False
 
@@ -475,8 +485,9 @@ class BatchEngine:
         t = self.traj.view(-1, self.traj_cap, 4)[s.p_off : s.p_off + s.p_count]
         return t
 
-    def fields(self, cid):
-        """The reference's u [X,Y,Z,3] and rho [X,Y,Z,1] of a case after its last step."""
+    def fields(self, cid, with_f=False):
+        """The reference's u [X,Y,Z,3] and rho [X,Y,Z,1] of a case after its
+        last step (and f [X,Y,Z,Q] with `with_f`)."""
         s = self.slots[cid]
         dev = self.device
         n = s.spec.n
@@ -485,13 +496,18 @@ class BatchEngine:
         esz = self.pool[0].element_size()
         u = torch.empty((3, n), device=dev, dtype=torch.float32)
         rho = torch.empty((n,), device=dev, dtype=torch.float32)
+        f = torch.empty((s.spec.q, n), device=dev, dtype=torch.float32) if with_f else None
         fol_new = self.fol[par][3 * s.node_off :]
         fol_old = self.fol[a][3 * s.node_off :]
-        stamp = self.hdr[H_STAMP : H_STAMP + 1]
+        # the stamp of the case's last spreading is its step count
+        stamp = self.case_done[cid : cid + 1]
         nat.call(
             "funlbm_materialize", s.lat, ctypes.c_void_p(self.pool[a].data_ptr() + s.f_off * esz),
             nat.ptr(fol_old), nat.ptr(fol_new), nat.ptr(self.fol_tiles[s.tile_off :]), nat.ptr(stamp),
-            None, None, nat.ptr(u), nat.ptr(rho), None, nat.stream_handle(dev),
+            None, nat.ptr(f), nat.ptr(u), nat.ptr(rho), None, nat.stream_handle(dev),
         )
         X, Y, Z = s.spec.nodes
-        return u.view(3, X, Y, Z).permute(1, 2, 3, 0), rho.view(X, Y, Z, 1)
+        out = (u.view(3, X, Y, Z).permute(1, 2, 3, 0), rho.view(X, Y, Z, 1))
+        if with_f:
+            out += (f.view(s.spec.q, X, Y, Z).permute(1, 2, 3, 0),)
+        return out

@@ -20,13 +20,13 @@
 
 namespace funlbm {
 
-// Device header (int32): counts of the work lists and the spreading stamp.
+// Device header (int32): counts of the work lists and of the launched steps.
 enum : int {
   H_TILES = 0,      // active tiles in tile_case / tile_local
   H_MARKERS = 1,    // markers in the marker arrays
   H_PARTICLES = 2,  // particles in the particle arrays
   H_CASES = 3,      // cases in the case arrays
-  H_STAMP = 4,      // stamp of the last completed spreading
+  H_STAMP = 4,      // steps launched (each case's spreadings carry its own stamps)
   H_QEND = 5,       // [5], [6], [7]: end of the D3Q13, D3Q15, D3Q19 tiles (the list is
                     // ordered by lattice; D3Q27 tiles run to hdr[H_TILES])
   H_CHUNKS = 8,     // 256-node chunks of the particles' moment boxes
@@ -49,6 +49,12 @@ struct BatchDyn {
   int32_t* done;  // [C] steps taken
   int32_t* flag;  // [C] weight-sum failure flag
 };
+
+// The spreading stamp of a case's coming step: its own step count plus one.
+// A case's tile entries and owner words carry its own stamps, so they do not
+// depend on how far other cases have stepped (the host zeroes both ranges
+// when it admits a case into them).
+__device__ inline uint32_t case_stamp(const BatchDyn& dyn, int c) { return (uint32_t)dyn.done[c] + 1u; }
 
 template <class Fn>
 __device__ inline void with_lattice(int q, Fn&& fn) {
@@ -75,12 +81,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_fused(
   constexpr int li = L::Q == 13 ? 0 : L::Q == 15 ? 1 : L::Q == 19 ? 2 : 3;
   const int t_begin = li == 0 ? 0 : hdr[H_QEND + li - 1];
   const int t_end = li == 3 ? hdr[H_TILES] : hdr[H_QEND + li];
-  const uint32_t stamp = (uint32_t)hdr[H_STAMP] + 1u;  // this step's spreading
   for (int t = t_begin + (int)blockIdx.x; t < t_end; t += gridDim.x) {
     const int c = tile_case[t];
     if (c < 0 || dyn.left[c] <= 0) continue;
     const CaseDesc& cd = cases[c];
     if (cd.q != L::Q) continue;
+    const uint32_t stamp = case_stamp(dyn, c);  // this step's spreading
     const int lt = tile_local[t];
     const int64_t node = (int64_t)lt * kBlock + threadIdx.x;
     if (node >= cd.p.n) continue;
@@ -216,7 +222,6 @@ __global__ __launch_bounds__(kIbmBlock) void k_batch_spread_prep(
     const int* __restrict__ pbox) {
   __shared__ PrepPeer peers[kIbmBlock / kWave][kMarkersPerWave];
   const int nm = hdr[H_MARKERS];
-  const uint32_t stamp = (uint32_t)hdr[H_STAMP] + 1u;
   const int sub = threadIdx.x & (kMarkerLanes - 1);
   const int wv = threadIdx.x / kWave;
   const int g = (threadIdx.x & (kWave - 1)) / kMarkerLanes;
@@ -236,7 +241,9 @@ __global__ __launch_bounds__(kIbmBlock) void k_batch_spread_prep(
     float* fn = nullptr;
     const int* org = nullptr;
     int key = 0;
+    uint32_t stamp = 0;
     if (act) {
+      stamp = case_stamp(dyn, c);
       const int par = dyn.par[c];
       // FOL_new of this step is buffer 1 - par; its origins were recorded the
       // last time the case wrote that buffer
@@ -260,10 +267,10 @@ __global__ __launch_bounds__(kClaimBlock) void k_batch_spread_claim(
     float* __restrict__ fol_b, unsigned long long* __restrict__ owner) {
   __shared__ int own[kClaimNodes];
   const int np = hdr[H_PARTICLES];
-  const uint32_t stamp = (uint32_t)hdr[H_STAMP] + 1u;
   for (int i = blockIdx.x; i < np; i += gridDim.x) {
     const int c = particle_case[i];
     if (c < 0 || dyn.left[c] <= 0) continue;
+    const uint32_t stamp = case_stamp(dyn, c);
     const int* row = pbox + 3 * i;
     if (!claims_in_box(row)) continue;
     const CaseDesc& cd = cases[c];
@@ -286,7 +293,6 @@ __global__ __launch_bounds__(kIbmBlock) void k_batch_spread_write(
     float* __restrict__ fol_a, float* __restrict__ fol_b,
     const unsigned long long* __restrict__ owner, uint32_t* __restrict__ fol_tiles) {
   const int nm = hdr[H_MARKERS];
-  const uint32_t stamp = (uint32_t)hdr[H_STAMP] + 1u;
   const int sub = threadIdx.x & (kMarkerLanes - 1);
   const int stride = gridDim.x * kMarkersPerBlock;
   for (int m = blockIdx.x * kMarkersPerBlock + (int)threadIdx.x / kMarkerLanes; m < nm; m += stride) {
@@ -297,7 +303,7 @@ __global__ __launch_bounds__(kIbmBlock) void k_batch_spread_write(
     float* fn = (par ? fol_a : fol_b) + 3 * cd.node_off;
     int* org = par ? origin_a : origin_b;
     spread_write_marker(cd.p, m, m - case_marker_begin[c], sub, lx, lF, lm, fn, owner + cd.node_off,
-                        org, fol_tiles + cd.tile_begin, stamp);
+                        org, fol_tiles + cd.tile_begin, case_stamp(dyn, c));
     if (sub == 0 && m == case_marker_begin[c]) atomicOr(&origin_ok[c], 1 << (1 - par));
   }
 }
@@ -329,8 +335,8 @@ __global__ __launch_bounds__(kIbmBlock) void k_batch_pose(const int* __restrict_
   }
 }
 
-// Case bookkeeping after a step: parity flips, counters move, the stamp of
-// the spreading just completed becomes current.
+// Case bookkeeping after a step: parity flips, counters move (a case's step
+// count is also the stamp of its last spreading), the launch count advances.
 __global__ void k_batch_advance(int* __restrict__ hdr, BatchDyn dyn) {
   const int nc = hdr[H_CASES];
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {

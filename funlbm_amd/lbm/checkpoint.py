@@ -58,6 +58,8 @@ def load(lbm, path: str) -> None:
             raise ValueError(f"checkpoint f has shape {f.shape}, the flow {tuple(fl.f.shape)}")
         if "flow/FOL" in z.files:
             fl.FOL = torch.from_numpy(np.ascontiguousarray(z["flow/FOL"]))
+        else:
+            fl.FOL = torch.zeros(tuple(fl.f.shape[:3]) + (3,))
         fl.f = torch.from_numpy(np.ascontiguousarray(f))
         fl.update_u_rou()
         if "flow/steps_done" in z.files and hasattr(fl, "steps_done"):

@@ -333,10 +333,13 @@ int funlbm_case_pack(const funlbm_lattice* lat, int64_t f_offset, int64_t node_o
 /* One timestep of every case whose steps-left counter is positive: the
  * single-lattice sequence (interp, wall, forces, spreading, fused step, pose)
  * as persistent kernels over device-side work lists, then the per-case
- * counters advance (parity flips, left -1, done +1) and hdr[4] (the spreading
- * stamp) +1.  hdr is int32[9] device memory: [0] active tiles in
+ * counters advance (parity flips, left -1, done +1) and hdr[4] +1.  A case's
+ * spreading stamps are its own step counts (done + 1 in the step), so its
+ * tile entries and owner words must be zero when it is admitted; after its
+ * last step, funlbm_materialize of the case takes ctl[0] = its done count.
+ * hdr is int32[9] device memory: [0] active tiles in
  * tile_case/tile_local, [1] marker slots, [2] particle slots, [3] case slots,
- * [4] stamp, [5], [6], [7] the ends of the D3Q13, D3Q15 and D3Q19 tiles: the
+ * [4] steps launched, [5], [6], [7] the ends of the D3Q13, D3Q15 and D3Q19 tiles: the
  * tile list holds the tiles of D3Q13 cases first, then D3Q15, D3Q19 and D3Q27
  * ones (one fused kernel per lattice takes its range), [8] entries of
  * chunk_particle.  A slot whose case index is -1 is skipped.  particle_markers

@@ -237,3 +237,119 @@ def test_diverging_case_stays_in_bounds_and_isolated():
     assert torch.equal(eng.fields(b)[0], ref.fields(c)[0])
     rows = eng.particle_rows(a)[0, :3].cpu().numpy()
     assert not np.all(np.abs(rows) < 1e3)  # it did diverge
+
+
+def test_batched_f32_case_matches_oracle():
+    """A batched float32 case against the oracle's run_step
+    (src/funlbm/lbm/base.py:146-179) directly: populations within 1e-5
+    relative L-inf, the centroid within 1e-6 lattice units after 30 steps
+    (the oracle keeps the particle state in double, as the engine does)."""
+    from funlbm_amd.batch.engine import markers_for
+    from oracle import ibm_oracle as ibo
+    from oracle import lbm_oracle as orc
+
+    spec = sphere_case((22, 20, 26), q=19, tau=0.62, r=3.5, rou=1.7, steps=30, center=(11.3, 10.2, 17.6))
+    eng = BatchEngine(storage="f32", max_nodes=1 << 16, max_markers=4096, max_particles=8, max_cases=4)
+    cid = eng.add(spec)
+    eng.step(30)
+    u, rho, f = eng.fields(cid, with_f=True)
+    st = orc.init_flow(19, spec.nodes, tau=np.float32(spec.tau), bc=[orc.WALL] * 6)
+    orc.init_solver(st)
+    p = spec.particles[0]
+    op = ibo.Particle(kind="sphere", r=p.r, rou=p.rou, center=p.center, angle=p.angle, pdtype=np.float64,
+                      lagrange=markers_for(p)[0], gravity=spec.gravity)
+    size = [n - 1 for n in spec.nodes]
+    for s in range(1, 31):
+        ibo.run_step(st, [op], size, dt=1.0, gl=0.0, step=s)
+    ef = float(np.max(np.abs(f.cpu().numpy() - st.f)) / np.max(np.abs(st.f)))
+    ec = float(np.max(np.abs(eng.particle_rows(cid)[0, 0:3].cpu().numpy() - op.cx)))
+    assert ef < 1e-5, ef
+    assert ec < 1e-6, ec
+    assert float(np.max(np.abs(op.cx - np.array(p.center)))) > 1e-5  # it moved
+
+
+def test_fields_of_a_case_finished_before_the_others():
+    """A case that finishes while others keep stepping materialises the same
+    fields as when solved alone: its spreading stamps are its own step
+    counts, so the FOL of its last two steps is still read after the others
+    have stepped on."""
+    short = sphere_case((24, 22, 26), steps=15, r=3.0)
+    long = CaseSpec(nodes=(18, 18, 18), q=15, tau=0.8, steps=40,
+                    particles=[ParticleSpec(kind="sphere", r=3.0, rou=2.0, center=(9, 9, 10))])
+    for storage in ("f32", "f16"):
+        eng = BatchEngine(storage=storage, max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8)
+        a = eng.add(short)
+        eng.add(long)
+        eng.step(40)
+        alone = BatchEngine(storage=storage, max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8)
+        b = alone.add(short)
+        alone.step(15)
+        ua, ra, fa = eng.fields(a, with_f=True)
+        ub, rb, fb = alone.fields(b, with_f=True)
+        assert torch.equal(ua, ub) and torch.equal(ra, rb) and torch.equal(fa, fb), storage
+        # the particle's force density is in the fields (not a state at rest)
+        assert float(ua.abs().max()) > 0
+
+
+def test_settling_batch_of_256_cases():
+    """The 256-case settling workload (sides log-uniform 32-96, D3Q15/19 by
+    coin flip, one sphere each, f16 deviation storage), 20 steps in one
+    engine: the smallest and largest case of each lattice equal their solves
+    alone bit for bit; the same 256 cases in float32 storage leave their
+    smallest case within 1e-5 of the oracle."""
+    from funlbm_amd.batch.engine import markers_for, prepare_markers
+    from funlbm_amd.batch.workloads import settling_batch
+    from oracle import ibm_oracle as ibo
+    from oracle import lbm_oracle as orc
+
+    steps = 20
+    specs = settling_batch(n_cases=256, seed=0, steps=steps)
+    prepare_markers([p for s in specs for p in s.particles])
+    nodes = sum(s.n for s in specs)
+    markers = sum(len(markers_for(p)[0]) for s in specs for p in s.particles)
+
+    def engine(storage, cases):
+        return BatchEngine(storage=storage, max_nodes=nodes + 256 * 256, max_markers=markers + 64,
+                           max_particles=256, max_cases=cases, max_q=19, traj_cap=8)
+
+    sample = []
+    for q in (15, 19):
+        ids = [i for i, s in enumerate(specs) if s.q == q]
+        sample += [min(ids, key=lambda i: specs[i].n), max(ids, key=lambda i: specs[i].n)]
+    eng = engine("f16", 256)
+    cids = [eng.add(s) for s in specs]
+    eng.step(5)  # eager, then the rest from a captured graph
+    eng.capture(steps - 5)
+    eng.run_graph()
+    torch.cuda.synchronize()
+    assert int(eng.case_done.max().item()) == steps and int(eng.case_left.max().item()) == 0
+    for i in sample:
+        one = BatchEngine(storage="f16", max_nodes=specs[i].n + 16384, max_markers=markers + 64,
+                          max_particles=4, max_cases=1, max_q=19, traj_cap=8)
+        c = one.add(specs[i])
+        one.step(steps)
+        um, rm = eng.fields(cids[i])
+        uo, ro = one.fields(c)
+        assert torch.equal(um, uo) and torch.equal(rm, ro), i
+        assert torch.equal(eng.particle_rows(cids[i])[:, :15], one.particle_rows(c)[:, :15]), i
+        del one
+    del eng
+    torch.cuda.empty_cache()
+    eng = engine("f32", 256)
+    cids = [eng.add(s) for s in specs]
+    eng.step(steps)
+    i = min(range(len(specs)), key=lambda k: specs[k].n)
+    spec = specs[i]
+    _, _, f = eng.fields(cids[i], with_f=True)
+    st = orc.init_flow(spec.q, spec.nodes, tau=np.float32(spec.tau), bc=[orc.WALL] * 6)
+    orc.init_solver(st)
+    p = spec.particles[0]
+    op = ibo.Particle(kind="sphere", r=p.r, rou=p.rou, center=p.center, angle=p.angle, pdtype=np.float64,
+                      lagrange=markers_for(p)[0], gravity=spec.gravity)
+    for s in range(1, steps + 1):
+        ibo.run_step(st, [op], [n - 1 for n in spec.nodes], dt=1.0, gl=0.0, step=s)
+    ef = float(np.max(np.abs(f.cpu().numpy() - st.f)) / np.max(np.abs(st.f)))
+    ec = float(np.max(np.abs(eng.particle_rows(cids[i])[0, 0:3].cpu().numpy() - op.cx)))
+    assert ef < 1e-5 and ec < 1e-6, (ef, ec)
+    del eng
+    torch.cuda.empty_cache()

@@ -270,7 +270,7 @@ def test_step_statistics_equal_separate_pass(q, storage):
 
 
 @pytest.mark.parametrize("size", [(11, 9, 13), (9, 10, 15)])
-@pytest.mark.parametrize("q", [15, 19, 27])
+@pytest.mark.parametrize("q", [13, 15, 19, 27])
 def test_fused_bitwise_equal_to_oracle(q, size):
     """Every quotient of the kernels is the correctly rounded one (three-
     operation division by a rounded reciprocal) and every sum runs in the
@@ -383,3 +383,40 @@ def test_neq_faces_refused_by_fused_entry_points():
         fl.begin_step()
     rc = nat.lib().funlbm_fused_step(ctypes.byref(fl.lat), None, None, None, None, None, None, None, None, None)
     assert rc == nat.EUNSUPPORTED_BC
+
+
+MOVING_SETS = {
+    # moving walls on the output (x) face and the top (z) face: the x = X,
+    # z = Z edge takes both rules, the top one winning for e_z = -1
+    "output_top": ([W, MW, P, P, W, MW], [None, [0.0, 0.02, -0.01], None, None, None, [0.05, 0.01, 0.0]]),
+    # moving back (y) and bottom (z) faces, walls opposite
+    "back_bottom": ([P, P, MW, W, MW, W], [None, None, [0.03, 0.0, 0.01], None, [-0.02, 0.04, 0.0], None]),
+}
+
+
+@pytest.mark.parametrize("faces", list(MOVING_SETS))
+@pytest.mark.parametrize("q", [15, 19, 27])
+def test_fused_moving_wall_bitwise_equal_to_oracle(q, faces):
+    """WALL_WITH_SPEED on the fused path (the default run_step): the x/y-face
+    rule through the per-node source, the z-face fix-up and the face density
+    of the previous step that both read, bit for bit with the oracle's
+    f_stream (src/funlbm/flow/d3/base.py:204-213,305-321) after 120 steps,
+    the corner nodes of two moving faces included."""
+    bcs, uw = MOVING_SETS[faces]
+    size = (9, 10, 13)
+    nodes = tuple(s + 1 for s in size)
+    f0 = random_state(q, nodes, seed=21)
+    lbm = run_solver(q, size, 0.06, bcs, 120, f0=f0, uw=uw)
+    assert lbm.flow.has_moving_wall
+    st = orc.init_flow(q, nodes, mu=0.06, bc=bcs, uw=uw)
+    orc.init_solver(st)
+    st.f = f0.copy()
+    orc.update_u_rou(st)
+    for _ in range(120):
+        orc.fluid_step(st)
+    got = lbm.flow.f.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), st.f.view(np.uint32)), rel_linf(got, st.f)
+    rho = lbm.flow.rou.cpu().numpy()
+    assert np.array_equal(rho.view(np.uint32), st.rou.view(np.uint32))
+    # the walls drive a flow: the test is not passing on a state at rest
+    assert float(np.abs(st.u).max()) > 1e-3

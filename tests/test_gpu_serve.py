@@ -62,9 +62,62 @@ def test_server_mix_equals_alone_and_respects_budget():
         alone.step(r.spec.steps)
         want = alone.particle_rows(cid).cpu().numpy()
         assert np.array_equal(r.result[:, :15], want[:, :15]), r.rid
-        # the streamed trajectory ends at the final centre
-        last = r.trajectory[-1]
-        slot = (r.spec.steps - 1) % eng.traj_cap
-        assert last[0, slot, 3] == r.spec.steps
-        assert np.allclose(last[:, slot, :3], want[:, 0:3].astype(np.float32))
+        # the streamed trajectory holds every step once, in order, and ends
+        # at the final centre
+        traj = np.concatenate(r.trajectory, axis=1)
+        assert np.array_equal(traj[0, :, 3], np.arange(1, r.spec.steps + 1, dtype=np.float32))
+        assert np.allclose(traj[:, -1, :3], want[:, 0:3].astype(np.float32))
     assert stats["p99_latency_s"] >= stats["p50_latency_s"] > 0
+
+
+def _engine_for(specs, storage="f16", traj_cap=32):
+    from funlbm_amd.batch.engine import markers_for
+
+    nodes = sum(s.n for s in specs)
+    markers = sum(len(markers_for(p)[0]) for s in specs for p in s.particles)
+    return BatchEngine(storage=storage, max_nodes=nodes + 4096 * len(specs), max_markers=markers + 64,
+                       max_particles=4 * len(specs) + 4, max_cases=len(specs) + 1, max_q=27, traj_cap=traj_cap)
+
+
+def test_serving_stream_mix_equals_alone_and_f16_error():
+    """A short Poisson stream from the serving workload's generator (sides
+    32-48, 300-600 steps; D3Q19 and D3Q27 requests with spheres and
+    ellipsoids), served with continuous batching: every result equals the
+    request solved alone, and on a D3Q27 request the f16 velocity field stays
+    within 1e-3 (relative L-inf) of the float64 solve, the centroid within
+    1e-5 lattice units."""
+    from funlbm_amd.batch.workloads import serving_requests
+
+    stream = serving_requests(40.0, 0.25, seed=0, side=(32, 48), steps=(300, 600))
+    specs = [s for _, s in stream]
+    assert {s.q for s in specs} == {19, 27}
+    assert {p.kind for s in specs for p in s.particles} == {"sphere", "ellipsoid"}
+    reqs = [Request(rid=i, spec=s, arrival=t) for i, (t, s) in enumerate(stream)]
+    eng = _engine_for(specs)
+    srv = Server(eng, hbm_budget=200e9, tick_steps=25)
+    stats = srv.run(reqs)
+    assert stats["completed"] == len(reqs) and stats["backlog"] == 0
+    for r in reqs:
+        alone = _engine_for([r.spec])
+        cid = alone.add(r.spec)
+        alone.step(r.spec.steps)
+        want = alone.particle_rows(cid).cpu().numpy()
+        assert np.array_equal(r.result[:, :15], want[:, :15]), r.rid
+        del alone
+    torch.cuda.empty_cache()
+    spec = next(s for s in specs if s.q == 27)
+    out = {}
+    for st in ("f64", "f16"):
+        e = _engine_for([spec], storage=st)
+        cid = e.add(spec)
+        e.step(spec.steps)
+        out[st] = (e.fields(cid)[0].double().cpu().numpy(), e.particle_rows(cid)[:, 0:3].cpu().numpy())
+        del e
+    u64, c64 = out["f64"]
+    u16, c16 = out["f16"]
+    err = float(np.abs(u16 - u64).max() / np.abs(u64).max())
+    cerr = float(np.abs(c16 - c64).max())
+    print(f"D3Q27 {spec.nodes} {len(spec.particles)} particles {spec.steps} steps: f16 u rel L-inf {err:.3e} "
+          f"(max|u| {np.abs(u64).max():.3e}), centroid {cerr:.3e}")
+    assert err <= 1e-3, err
+    assert cerr <= 1e-5, cerr
