@@ -22,7 +22,7 @@ BC_WALL = 1200
 BC_WALL_WITH_SPEED = 1201
 BC_NON_EQUILIBRIUM = 1400
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 ROUND_NEAREST = 0
 ROUND_STOCHASTIC = 1
@@ -92,6 +92,7 @@ SIGNATURES = {
     "funlbm_init_state": (c_int32, [LATP, P, P]),
     "funlbm_fused_step": (c_int32, [LATP, P, P, P, P, P, P, P, P, P]),
     "funlbm_fused_step_stats": (c_int32, [LATP, P, P, P, P, P, P, P, P, P, c_int32, P, P]),
+    "funlbm_fused_step_stats_noalloc": (c_int32, [LATP, P, P, P, P, P, P, P, P, P, P]),
     "funlbm_materialize": (c_int32, [LATP, P, P, P, P, P, P, P, P, P, P, P]),
     "funlbm_copy_calibrate": (c_int32, [P, P, c_int64, P]),
     "funlbm_ibm_interp": (

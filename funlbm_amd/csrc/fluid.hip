@@ -3,6 +3,8 @@
 // Layout: populations f[k][x][y][z] (z fastest), one thread per node over a
 // flattened 1-D grid of 256-thread blocks, so rows of any length keep every
 // lane busy and every access to a direction plane is a coalesced row.
+#include <mutex>
+
 #include "capi_util.h"
 #include "fused_node.h"
 #include "lbm_common.h"
@@ -517,26 +519,27 @@ int funlbm_fused_step(const funlbm_lattice* lat, const void* state_src, void* st
                       nullptr, nullptr, nullptr, nullptr, false, stream);
 }
 
-int funlbm_fused_step_stats(const funlbm_lattice* lat, const void* state_src, void* state_dst,
-                            const float* fol_old, const float* fol_new, const uint32_t* fol_tiles,
-                            const int32_t* ctl, const float* rho_prev, float* rho_next, double* partials,
-                            int32_t n_partials, double* out, void* stream) {
+// The logged fused step with its block partials in `partials` (n_partials
+// entries of kStats doubles); `who` names the entry point in errors.
+static int fused_stats_launch(const funlbm_lattice* lat, const void* state_src, void* state_dst,
+                              const float* fol_old, const float* fol_new, const uint32_t* fol_tiles,
+                              const int32_t* ctl, const float* rho_prev, float* rho_next, double* partials,
+                              int64_t n_partials, double* out, void* stream, const char* who) {
   KParams p;
   int rc = make_params(lat, &p);
   if (rc) return rc;
-  if ((rc = reject_neq(p, "fused_step_stats"))) return rc;
-  if (!state_src || !state_dst) return set_error(FUNLBM_EINVAL, "fused: NULL state");
-  if (state_src == state_dst) return set_error(FUNLBM_EINVAL, "fused: source and destination alias");
-  if (!partials || !out) return set_error(FUNLBM_EINVAL, "fused_step_stats: NULL statistics array");
+  if ((rc = reject_neq(p, who))) return rc;
+  if (!state_src || !state_dst) return set_error(FUNLBM_EINVAL, "%s: NULL state", who);
+  if (state_src == state_dst) return set_error(FUNLBM_EINVAL, "%s: source and destination alias", who);
+  if (!partials || !out) return set_error(FUNLBM_EINVAL, "%s: NULL statistics array", who);
   if (p.has_moving && !rho_prev)
-    return set_error(FUNLBM_EINVAL, "fused: moving wall needs the previous face density");
-  if (fol_tiles && !ctl) return set_error(FUNLBM_EINVAL, "fused: a FOL tile table needs ctl");
+    return set_error(FUNLBM_EINVAL, "%s: moving wall needs the previous face density", who);
+  if (fol_tiles && !ctl) return set_error(FUNLBM_EINVAL, "%s: a FOL tile table needs ctl", who);
   const int blocks = grid_for(p.n);
   const int stage = std::min(blocks, kStatsStage);
-  if ((int64_t)n_partials < (int64_t)blocks + stage)
-    return set_error(FUNLBM_EINVAL,
-                     "fused_step_stats: %d partials given, %d needed (one per 256 nodes, and %d more)", n_partials,
-                     blocks + stage, stage);
+  if (n_partials < (int64_t)blocks + stage)
+    return set_error(FUNLBM_EINVAL, "%s: %lld partials given, %d needed (one per 256 nodes, and %d more)", who,
+                     (long long)n_partials, blocks + stage, stage);
   const bool force = fol_old || fol_new || p.has_body;
   hipStream_t s = (hipStream_t)stream;
   rc = dispatch_lattice(lat->q, [&](auto l) {
@@ -548,7 +551,7 @@ int funlbm_fused_step_stats(const funlbm_lattice* lat, const void* state_src, vo
         hipLaunchKernelGGL(kern, dim3(blocks), dim3(kBlock), 0, s, p, (const E*)state_src, (E*)state_dst, fol_old,
                            fol_new, fol_tiles, ctl, rho_prev, rho_next, nullptr, nullptr, nullptr, nullptr,
                            partials);
-        return check_launch("fused_step_stats");
+        return check_launch(who);
       };
       return force ? go(k_fused_stats<L, S, true>) : go(k_fused_stats<L, S, false>);
     });
@@ -557,10 +560,71 @@ int funlbm_fused_step_stats(const funlbm_lattice* lat, const void* state_src, vo
   // many partials: a first stage over `stage` blocks, then one block
   double* second = partials + (int64_t)blocks * kStats;
   hipLaunchKernelGGL(k_stats_stage, dim3(stage), dim3(kBlock), 0, s, partials, blocks, second);
-  if ((rc = check_launch("fused_step_stats"))) return rc;
+  if ((rc = check_launch(who))) return rc;
   hipLaunchKernelGGL(k_stats_final, dim3(1), dim3(kBlock), 0, s, second, stage, (double)p.n * lat->q,
                      (double)p.n * 3, (double)p.n, out);
-  return check_launch("fused_step_stats");
+  return check_launch(who);
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of funlbm_fused_step_stats_noalloc: the block partials of a logged
+// step of up to kNoAllocNodes nodes (one partial per 256 nodes and
+// kStatsStage more, kStats doubles each: 4.5 MiB), in a file-scope device
+// array that each GPU holds for the life of the process.  Host code never
+// passes the array's name to a kernel (in host code it is a stand-in, not a
+// device address): the address comes from hipGetSymbolAddress, once per
+// device with that device current, and is kept per device.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNoAllocNodes = (int64_t)1 << 24;  // 256^3
+constexpr int64_t kNoAllocPartials = kNoAllocNodes / kBlock + kStatsStage;
+static __device__ __align__(256) double scratch[kNoAllocPartials * kStats];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static double* g_scratch[kMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up on its first use.
+static int scratch_address(double** out, const char* who) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_error((int)e, "%s: hipGetDevice: %s", who, hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return set_error(FUNLBM_EINVAL, "%s: device %d beyond %d", who, dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[dev]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess)
+      return set_error((int)e, "%s: hipGetSymbolAddress of the scratch array: %s", who, hipGetErrorString(e));
+    if (!ptr) return set_error(FUNLBM_EFAILED, "%s: the scratch array has no device address", who);
+    g_scratch[dev] = (double*)ptr;
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
+
+int funlbm_fused_step_stats(const funlbm_lattice* lat, const void* state_src, void* state_dst,
+                            const float* fol_old, const float* fol_new, const uint32_t* fol_tiles,
+                            const int32_t* ctl, const float* rho_prev, float* rho_next, double* partials,
+                            int32_t n_partials, double* out, void* stream) {
+  return fused_stats_launch(lat, state_src, state_dst, fol_old, fol_new, fol_tiles, ctl, rho_prev, rho_next,
+                            partials, n_partials, out, stream, "fused_step_stats");
+}
+
+int funlbm_fused_step_stats_noalloc(const funlbm_lattice* lat, const void* state_src, void* state_dst,
+                                    const float* fol_old, const float* fol_new, const uint32_t* fol_tiles,
+                                    const int32_t* ctl, const float* rho_prev, float* rho_next, double* out,
+                                    void* stream) {
+  const char* who = "fused_step_stats_noalloc";
+  KParams p;
+  int rc = make_params(lat, &p);
+  if (rc) return rc;
+  if (p.n > kNoAllocNodes)
+    return set_error(FUNLBM_ETOOBIG, "%s: %lld nodes exceed the %lld-node maximum of its scratch array", who,
+                     (long long)p.n, (long long)kNoAllocNodes);
+  double* partials = nullptr;
+  if ((rc = scratch_address(&partials, who))) return rc;
+  return fused_stats_launch(lat, state_src, state_dst, fol_old, fol_new, fol_tiles, ctl, rho_prev, rho_next,
+                            partials, kNoAllocPartials, out, stream, who);
 }
 
 int funlbm_materialize(const funlbm_lattice* lat, const void* state_src, const float* fol_old,

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define FUNLBM_ABI_VERSION 3
+#define FUNLBM_ABI_VERSION 4
 
 /* Boundary codes: the values of the reference enum BoundaryCondition
  * (src/funlbm/config/base.py:22-28). */
@@ -185,6 +185,22 @@ int funlbm_fused_step_stats(const funlbm_lattice* lat, const void* state_src, vo
                             const float* fol_old, const float* fol_new, const uint32_t* fol_tiles,
                             const int32_t* ctl, const float* rho_prev, float* rho_next, double* partials,
                             int32_t n_partials, double* out, void* stream);
+
+/* funlbm_fused_step_stats without caller-provided scratch, for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  No counterpart in the reference.
+ * The block partials live in a device array inside the library, sized for
+ * lattices of at most 2^24 nodes (256^3; 4.5 MiB, held by each GPU the
+ * library runs on until the process ends); a larger lattice is
+ * FUNLBM_ETOOBIG.  It allocates nothing.  The first call on a device looks up
+ * the array's address (hipGetSymbolAddress): make it outside a graph capture.
+ * Every call on one GPU shares that GPU's array, so calls that overlap on one
+ * GPU (other streams, other threads) race; calls in one stream do not.  The
+ * results equal funlbm_fused_step_stats's bit for bit. */
+int funlbm_fused_step_stats_noalloc(const funlbm_lattice* lat, const void* state_src, void* state_dst,
+                                    const float* fol_old, const float* fol_new, const uint32_t* fol_tiles,
+                                    const int32_t* ctl, const float* rho_prev, float* rho_next, double* out,
+                                    void* stream);
 
 /* The same pass as funlbm_fused_step without the collision: writes the
  * reference's f, u, rho and p (float) after the step.  Any output may be NULL. */

@@ -36,7 +36,7 @@ def test_bindings_cover_header():
 
 def test_abi_version_and_struct_size():
     lib = nat.lib()
-    assert lib.funlbm_abi_version() == 3
+    assert lib.funlbm_abi_version() == 4
     # 5 ints + 6 ints + 1 int + 3 floats + 3 floats + 18 floats + rounding + seed
     assert ctypes.sizeof(nat.Lattice) == 4 * (5 + 6 + 1 + 3 + 3 + 18 + 2)
 
@@ -191,3 +191,20 @@ def test_boxed_interp_argument_errors_need_no_device():
     assert f(*args(cache=ctypes.c_void_p(4096 + 4))) == nat.EINVAL
     assert "aligned" in nat.last_error()
     assert f(*args(dt=0.0)) == nat.EINVAL
+
+
+def test_noalloc_step_rejects_lattices_beyond_its_scratch():
+    """The allocation-free logged step holds partials for 2^24 nodes; a
+    larger lattice is refused on the host before any device call."""
+    lat = nat.Lattice()
+    lat.q = 19
+    lat.nx = lat.ny = lat.nz = 257
+    lat.tau = 0.6
+    lat.rho0 = 1.0
+    lat.dev_scale = 1024.0
+    for i in range(6):
+        lat.bc[i] = nat.BC_PERIODICAL
+    rc = nat.lib().funlbm_fused_step_stats_noalloc(ctypes.byref(lat), None, None, None, None, None, None, None,
+                                                   None, None, None)
+    assert rc == nat.ETOOBIG
+    assert "maximum" in nat.last_error()
