@@ -8,7 +8,11 @@ collision pass over the lattice.  Inputs are resident in HBM before timing.
 
 Prints one JSON line with MLUPS, the HBM roofline of the fused kernel (HIP
 events on the launch stream; algorithmic bytes 2*Q*sizeof(storage) per node)
-and a CPU baseline (the numpy oracle on a bounded sample).
+and a CPU baseline (the reference's torch op sequence on the host's threads,
+a bounded sample at 256^3).  Further legs: f16/bf16 deviation storage, D3Q27,
+the 64-ellipsoid suspension (timing and, at 2000 steps, its precision against
+float64), 256 batched settling cases against one-at-a-time solves, and a
+serving sweep over arrival rates.
 """
 
 import argparse
@@ -214,8 +218,18 @@ def copy_bandwidth(nbytes=2 << 30, reps=10):
     return 2.0 * nbytes / (ms * 1e-3) / 1e9
 
 
-def cpu_baseline(n=64, q=19, budget_s=15.0):
-    """The numpy oracle's fluid step on an n^3 periodic lattice (bounded)."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def numpy_oracle_rate(n=64, q=19, budget_s=8.0):
+    """The numpy oracle's fluid step on an n^3 periodic lattice, one core (bounded)."""
     from oracle import lbm_oracle as orc
 
     nodes = (n, n, n)
@@ -234,17 +248,56 @@ def cpu_baseline(n=64, q=19, budget_s=15.0):
         el = time.perf_counter() - t0
         if el > budget_s or steps >= 200:
             break
+    return {"value": round(n ** 3 * steps / el / 1e6, 3), "unit": "MLUPS", "cores": 1,
+            "sample": f"numpy oracle fluid step, D3Q{q} float32, {n}^3 periodic, {steps} steps in {el:.1f} s"}
+
+
+def cpu_baseline(n=256, q=19, budget_s=12.0):
+    """The reference's op sequence on CPU torch (oracle/torch_ref.py: matmul,
+    norm, roll, the per-direction forcing loop) at the headline lattice, on
+    all the threads torch uses on this host, timed for about budget_s after
+    one warm-up step; the numpy oracle on one core beside it."""
+    import torch
+
+    from oracle import lbm_oracle as orc
+    from oracle.torch_ref import TorchFlow
+
+    nodes = (n, n, n)
+    tf = TorchFlow(q, nodes, tau=np.float32(0.6), bc=[orc.PERIODICAL] * 6)
+    g = torch.Generator().manual_seed(0)
+    tf.rou = 1.0 + 1e-3 * torch.randn(nodes + (1,), generator=g)
+    tf.u = 1e-3 * torch.randn(nodes + (3,), generator=g)
+    tf.cul_equ()
+    tf.f = tf.feq.clone()  # f = feq(rho, u)
+    tf.update_u_rou()
+    t0 = time.perf_counter()
+    tf.step()  # warm
+    warm = time.perf_counter() - t0
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        tf.step()
+        steps += 1
+        el = time.perf_counter() - t0
+        if el + el / steps > budget_s:
+            break
+    del tf
     mlups = n ** 3 * steps / el / 1e6
     return {
         "value": round(mlups, 3),
         "unit": "MLUPS",
-        "cores": 1,
+        "cores": torch.get_num_threads(),
         "kind": "port",
-        "sample": f"numpy oracle fluid step, D3Q{q} float32, {n}^3 periodic, {steps} steps in {el:.1f} s",
+        "sample": f"the reference's torch op sequence (oracle/torch_ref.py), D3Q{q} float32, {n}^3 periodic, "
+                  f"tau 0.6, {steps} steps in {el:.1f} s after a {warm:.1f} s warm-up step",
+        "torch_threads": torch.get_num_threads(),
+        "os_cpu_count": os.cpu_count(),
+        "cpu_model": cpu_model(),
+        "numpy_oracle": numpy_oracle_rate(),
     }
 
 
-def bench_batch(n_cases=256, steps=3000, seed=0, storage="f16", tick=50, baseline_cases=3,
+def bench_batch(n_cases=256, steps=3000, seed=0, storage="f16", tick=50, baseline_per_lattice=8,
                 baseline_steps=200):
     """Batched independent settling cases (all cases advance in one persistent
     launch sequence per step, replayed from a HIP graph) against plain
@@ -283,9 +336,18 @@ def bench_batch(n_cases=256, steps=3000, seed=0, storage="f16", tick=50, baselin
     wall = time.perf_counter() - t0
     assert all(eng.left()[c] == 0 for c in eng.slots)
     lups = sum(s.n * s.steps for s in specs)
-    # one case at a time: the single-lattice solver, eager, same storage
-    per_case = []
-    for s in specs[:baseline_cases]:
+    # one case at a time: the single-lattice solver, eager, same storage, on
+    # a sample of each lattice's cases at evenly spaced size quantiles
+    # (smallest and largest included); the mean solve time weights each
+    # lattice's sample mean by its share of the cases
+    sample = []
+    for q in sorted({s.q for s in specs}):
+        ids = sorted((i for i, s in enumerate(specs) if s.q == q), key=lambda i: specs[i].n)
+        k = min(baseline_per_lattice, len(ids))
+        picks = sorted({ids[round(j * (len(ids) - 1) / max(k - 1, 1))] for j in range(k)})
+        sample.append((q, len(ids), picks))
+    per_case = {}
+    for s in [specs[i] for _, _, picks in sample for i in picks]:
         p = s.particles[0]
         faces = {f: {"code": "WALL"} for f in ("input", "output", "back", "forward", "bottom", "top")}
         cfg = Config().from_json({
@@ -306,9 +368,13 @@ def bench_batch(n_cases=256, steps=3000, seed=0, storage="f16", tick=50, baselin
             lbm.step = k
             lbm.run_step()
         torch.cuda.synchronize()
-        per_case.append((time.perf_counter() - t0) / baseline_steps * s.steps)
+        per_case[id(s)] = (time.perf_counter() - t0) / baseline_steps * s.steps
         del lbm
-    base_solves = 1.0 / (sum(per_case) / len(per_case))
+    mean_s = sum(cnt / len(specs) * float(np.mean([per_case[id(specs[i])] for i in picks]))
+                 for _, cnt, picks in sample)
+    base_solves = 1.0 / mean_s
+    n_base = sum(len(p) for _, _, p in sample)
+    sizes = [specs[i].n for _, _, p in sample for i in p]
     solves = n_cases / t
     return {
         "workload": f"{n_cases} settling cases, sides log-uniform 32-96, D3Q15/19, one sphere each, "
@@ -323,16 +389,20 @@ def bench_batch(n_cases=256, steps=3000, seed=0, storage="f16", tick=50, baselin
         "solves_per_s": round(solves, 2),
         "mlups": round(lups / t / 1e6, 1),
         "one_at_a_time_solves_per_s": round(base_solves, 3),
-        "one_at_a_time_sample": f"{baseline_cases} cases x {baseline_steps} steps with LBMD3, scaled to {steps} steps",
+        "one_at_a_time_sample": f"{n_base} cases ({', '.join(f'{len(p)} D3Q{q}' for q, _, p in sample)}; "
+                                f"{min(sizes)} to {max(sizes)} nodes, evenly spaced size quantiles) x "
+                                f"{baseline_steps} steps with LBMD3, scaled to {steps} steps, weighted by lattice",
         "speedup_vs_one_at_a_time": round(solves / base_solves, 2),
         "grid": dict(eng.grid),
         "autotune": tune,
     }
 
 
-def bench_serve(rate=20.0, duration=10.0, seed=0, steps=(1000, 20000), budget=200e9, storage="f16",
-                tick=50, pool_nodes=None):
-    """Poisson stream of settling requests served with continuous batching."""
+def serve_rate(rate, duration, seed=0, steps=(1000, 20000), budget=200e9, storage="f16", tick=50,
+               drain_s=6.0, pool_nodes_cap=1 << 28):
+    """Poisson stream of settling requests at `rate` req/s for `duration` s,
+    served with continuous batching; the run is cut `drain_s` after the last
+    arrival (requests still queued or running then are the backlog)."""
     import torch
 
     from funlbm_amd.batch.engine import BatchEngine, markers_for, prepare_markers
@@ -342,19 +412,36 @@ def bench_serve(rate=20.0, duration=10.0, seed=0, steps=(1000, 20000), budget=20
     reqs = serving_requests(rate, duration, seed=seed, steps=steps)
     prepare_markers([p for _, s in reqs for p in s.particles])
     need_nodes = sum(s.n for _, s in reqs)
-    pool_nodes = pool_nodes or min(need_nodes + 256 * len(reqs), 1 << 30)
+    pool_nodes = min(need_nodes + 4096 * len(reqs), pool_nodes_cap)
     markers = sum(len(markers_for(p)[0]) for _, s in reqs for p in s.particles)
     eng = BatchEngine(storage=storage, max_nodes=pool_nodes, max_markers=markers + 64,
                       max_particles=4 * len(reqs) + 4, max_cases=len(reqs) + 1, max_q=27, traj_cap=tick)
     srv = Server(eng, hbm_budget=budget, tick_steps=tick)
-    stats = srv.run([Request(rid=i, spec=s, arrival=t) for i, (t, s) in enumerate(reqs)])
+    stats = srv.run([Request(rid=i, spec=s, arrival=t) for i, (t, s) in enumerate(reqs)],
+                    max_seconds=duration + drain_s)
     del srv, eng
     torch.cuda.empty_cache()
-    # error of the served storage against float64 on a sample request solved alone
-    spec = reqs[0][1]
+    out = {"rate": rate}
+    for k in ("submitted", "completed", "backlog", "solves_per_s", "mlups", "p50_latency_s", "p99_latency_s",
+              "wall_s", "wait_s", "ticks"):
+        v = stats.get(k)
+        out[k] = round(v, 3) if isinstance(v, float) else v
+    return out
+
+
+def serve_error_check(seed=0, steps=(1000, 20000), storage="f16"):
+    """The served storage's error against float64 on request 0 of the
+    stream, solved alone (float32 storage beside it)."""
+    import torch
+
+    from funlbm_amd.batch.engine import BatchEngine, markers_for
+    from funlbm_amd.batch.workloads import serving_requests
+
+    spec = serving_requests(20.0, 1.0, seed=seed, steps=steps)[0][1]
     fields = {}
     for st in ("f64", "f32", storage):
-        e = BatchEngine(storage=st, max_nodes=spec.n + 16384, max_markers=sum(len(markers_for(p)[0]) for p in spec.particles) + 8,
+        e = BatchEngine(storage=st, max_nodes=spec.n + 16384,
+                        max_markers=sum(len(markers_for(p)[0]) for p in spec.particles) + 8,
                         max_particles=len(spec.particles) + 1, max_cases=1, max_q=27, traj_cap=8)
         cid = e.add(spec)
         e.step(spec.steps)
@@ -366,7 +453,7 @@ def bench_serve(rate=20.0, duration=10.0, seed=0, steps=(1000, 20000), budget=20
     us, cs = fields[storage]
     u32, c32 = fields["f32"]
     scale = float(np.abs(u64).max())
-    stats["error_check"] = {
+    return {
         "sample": f"request 0 alone: {spec.nodes[0]}^3 D3Q{spec.q}, {len(spec.particles)} particles, {spec.steps} steps",
         "max_abs_u_f64": scale,
         "u_rel_linf_vs_f64": float(np.abs(us - u64).max() / scale) if scale > 0 else None,
@@ -376,15 +463,42 @@ def bench_serve(rate=20.0, duration=10.0, seed=0, steps=(1000, 20000), budget=20
         "f32_u_rel_linf_vs_f64": float(np.abs(u32 - u64).max() / scale) if scale > 0 else None,
         "f32_centroid_abs_vs_f64": float(np.abs(c32 - c64).max()) if len(c64) else None,
     }
-    stats.update({
-        "workload": f"Poisson {rate} req/s for {duration} s, sides 32-128, D3Q19/27, 1-4 particles, "
-                    f"steps {steps[0]}-{steps[1]}, {storage}-deviation storage, {budget / 1e9:.0f} GB budget",
-        "rate": rate,
-    })
-    for k in ("wall_s", "busy_s", "solves_per_s", "mlups", "p50_latency_s", "p99_latency_s"):
-        if stats.get(k) is not None:
-            stats[k] = round(stats[k], 3)
-    return stats
+
+
+def bench_serve(rates, duration, steps=(1000, 20000), budget=200e9, storage="f16", tick=50, drain_s=6.0):
+    sweep = [serve_rate(r, duration, steps=steps, budget=budget, storage=storage, tick=tick, drain_s=drain_s)
+             for r in rates]
+    return {
+        "workload": f"Poisson arrivals for {duration} s per rate (run cut {drain_s} s after the last arrival), "
+                    f"sides log-uniform 32-128, D3Q19/27, 1-4 spheres or ellipsoids, steps {steps[0]}-{steps[1]}, "
+                    f"{storage}-deviation storage, {budget / 1e9:.0f} GB budget, ticks of {tick} steps, "
+                    f"two ticks in flight",
+        "sweep": sweep,
+        "error_check": serve_error_check(steps=steps, storage=storage),
+    }
+
+
+def bench_suspension_precision(size=256, n_particles=64, steps=2000):
+    """The dense-suspension precision study at its configuration: D3Q27
+    periodic box, 64 prolate ellipsoids (4, 4, 8), density ratio 1.2, body
+    force 1e-6 along x, tau 0.55; float32, f16 and bf16 storage (both
+    roundings) against float64 after `steps` steps.  Also with u not zeroed
+    on the (periodic) faces, which the reference does after every moment
+    update."""
+    from funlbm_amd import precision
+
+    out = {}
+    for name, zero in (("reference_faces", None), ("no_face_zeroing", [])):
+        cfg = precision.suspension(n=size, n_particles=n_particles, mu=0.05 / 3)
+        if zero is not None:
+            cfg["flow"]["uzero_faces"] = zero
+        t0 = time.perf_counter()
+        res = precision.compare(cfg, steps)
+        res["seconds"] = round(time.perf_counter() - t0, 1)
+        out[name] = res
+    out["workload"] = (f"D3Q27 {size}^3 periodic, {n_particles} ellipsoids (4,4,8), rho_p 1.2, body force 1e-6 x, "
+                       f"tau 0.55, {steps} steps; u rel L-inf and centroid error vs float64")
+    return out
 
 
 def main():
@@ -400,15 +514,18 @@ def main():
     ap.add_argument("--batch-cases", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=3000)
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--serve-rate", type=float, default=20.0)
-    ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-seconds", type=float, default=4.0,
+                    help="seconds of arrivals per rate")
     ap.add_argument("--serve-steps", default="1000,20000", help="min,max steps of a request")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-f16", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the bf16, D3Q27 and suspension legs")
-    ap.add_argument("--serve-rates", default="",
-                    help="comma-separated extra arrival rates (req/s) for a serving sweep")
+    ap.add_argument("--serve-rates", default="10,20,50,100,200",
+                    help="comma-separated arrival rates (req/s) of the serving sweep")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the D3Q27 suspension precision study")
+    ap.add_argument("--precision-steps", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -507,20 +624,19 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_serve:
         lo, hi = (int(v) for v in args.serve_steps.split(","))
+        rates = [float(v) for v in args.serve_rates.split(",") if v]
         try:
-            out["serving"] = bench_serve(rate=args.serve_rate, duration=args.serve_seconds, steps=(lo, hi))
-        except Exception as e:
+            out["serving"] = bench_serve(rates, args.serve_seconds, steps=(lo, hi))
+        except Exception as e:  # reported, never silent
             out["serving"] = {"error": repr(e)}
         torch.cuda.empty_cache()
-    if args.serve_rates:
-        lo, hi = (int(v) for v in args.serve_steps.split(","))
-        sweep = []
-        for r in (float(v) for v in args.serve_rates.split(",")):
-            s = bench_serve(rate=r, duration=args.serve_seconds, steps=(lo, hi))
-            sweep.append({k: s[k] for k in ("rate", "completed", "solves_per_s", "p50_latency_s", "p99_latency_s",
-                                            "busy_s", "wall_s")})
-            torch.cuda.empty_cache()
-        out["serving_sweep"] = sweep
+    if not args.no_precision:
+        try:
+            out["suspension_precision"] = bench_suspension_precision(size=args.size,
+                                                                     steps=args.precision_steps)
+        except Exception as e:  # reported, never silent
+            out["suspension_precision"] = {"error": repr(e)}
+        torch.cuda.empty_cache()
     if not args.no_copy:
         cb = copy_bandwidth()
         out["roofline"]["copy_peak_gbs"] = round(cb, 1)

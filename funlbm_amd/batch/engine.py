@@ -272,7 +272,7 @@ class BatchEngine:
         desc = (ctypes.c_uint8 * self.case_bytes)()
         nat.call("funlbm_case_pack", lat, f_off, node_off, tile_off, desc)
         self.cases[cid * self.case_bytes : (cid + 1) * self.case_bytes] = self._h2d(
-            np.frombuffer(bytes(desc), dtype=np.uint8))
+            np.frombuffer(bytes(desc), dtype=np.uint8).copy())
         # state: the reference's init (f = w, its moments, the two collisions
         # before the first stream), as LBMD3 enters its fused steps
         esz = self.pool[0].element_size()
