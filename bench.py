@@ -347,6 +347,8 @@ def bench_batch(n_cases=256, steps=3000, seed=0, storage="f16", tick=50, baselin
         picks = sorted({ids[round(j * (len(ids) - 1) / max(k - 1, 1))] for j in range(k)})
         sample.append((q, len(ids), picks))
     per_case = {}
+    if baseline_per_lattice <= 0:
+        sample = []
     for s in [specs[i] for _, _, picks in sample for i in picks]:
         p = s.particles[0]
         faces = {f: {"code": "WALL"} for f in ("input", "output", "back", "forward", "bottom", "top")}
@@ -372,9 +374,9 @@ def bench_batch(n_cases=256, steps=3000, seed=0, storage="f16", tick=50, baselin
         del lbm
     mean_s = sum(cnt / len(specs) * float(np.mean([per_case[id(specs[i])] for i in picks]))
                  for _, cnt, picks in sample)
-    base_solves = 1.0 / mean_s
+    base_solves = 1.0 / mean_s if sample else float("nan")
     n_base = sum(len(p) for _, _, p in sample)
-    sizes = [specs[i].n for _, _, p in sample for i in p]
+    sizes = [specs[i].n for _, _, p in sample for i in p] or [0]
     solves = n_cases / t
     return {
         "workload": f"{n_cases} settling cases, sides log-uniform 32-96, D3Q15/19, one sphere each, "
@@ -514,6 +516,8 @@ def main():
     ap.add_argument("--batch-cases", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=3000)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-baseline-per-lattice", type=int, default=8,
+                    help="one-at-a-time sample cases per lattice (0: no baseline)")
     ap.add_argument("--serve-seconds", type=float, default=4.0,
                     help="seconds of arrivals per rate")
     ap.add_argument("--serve-steps", default="1000,20000", help="min,max steps of a request")
@@ -618,7 +622,8 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_batch:
         try:
-            out["batched"] = bench_batch(n_cases=args.batch_cases, steps=args.batch_steps)
+            out["batched"] = bench_batch(n_cases=args.batch_cases, steps=args.batch_steps,
+                                         baseline_per_lattice=args.batch_baseline_per_lattice)
         except Exception as e:  # reported, never silent
             out["batched"] = {"error": repr(e)}
         torch.cuda.empty_cache()

@@ -115,7 +115,7 @@ SIGNATURES = {
     "funlbm_flow_conservation": (c_int32, [LATP, P, P, P, P, P, P, P, c_int32, P, P]),
     "funlbm_case_bytes": (c_int32, []),
     "funlbm_case_pack": (c_int32, [LATP, c_int64, c_int64, c_int32, P]),
-    "funlbm_batch_step": (c_int32, [c_int32] + [P] * 30 + [c_int32] * 4 + [P, P, P, c_int32, P, c_int32, P]),
+    "funlbm_batch_step": (c_int32, [c_int32] + [P] * 30 + [c_int32] * 4 + [P, P, P, c_int32, P, c_int32, P, P]),
 }
 
 _lib = None

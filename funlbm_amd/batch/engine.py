@@ -25,6 +25,8 @@ from ..particle.markers import surface_markers
 from .alloc import RangeAllocator
 
 TILE = 256
+# smallest grid of a lattice's fused kernel (8 blocks per compute unit)
+MIN_LATTICE_GRID = 2048
 H_TILES, H_MARKERS, H_PARTICLES, H_CASES, H_STAMP = 0, 1, 2, 3, 4
 H_CHUNKS, H_WORDS = 8, 9
 
@@ -214,6 +216,7 @@ class BatchEngine:
         self._graph_steps = 0
         self._graph_mask = 0
         self._mask = 0  # lattices with cases (bit i: D3Q13, 15, 19, 27), bit 4: per-marker claims
+        self._lattice_tiles = [0, 0, 0, 0]
         self._dirty = True
 
     def _h2d(self, arr):
@@ -361,6 +364,7 @@ class BatchEngine:
                     tl.append(np.arange(s.n_tiles, dtype=np.int32))
             ends.append(int(sum(len(a) for a in tc)))
         nt = ends[-1]
+        self._lattice_tiles = [b - a for a, b in zip([0] + ends[:-1], ends)]
         self._mask = sum(1 << i for i, q in enumerate((13, 15, 19, 27))
                          if any(s.spec.q == q for s in self.slots.values()))
         # per-marker spreading claims while a particle has no shared-memory box
@@ -388,6 +392,14 @@ class BatchEngine:
         self._dirty = False
 
     # ------------------------------------------------------------------ stepping
+    def _lattice_grid(self):
+        """Grid of each lattice's fused kernel: its own tile count, at least
+        MIN_LATTICE_GRID blocks (cases admitted after a graph capture are
+        then still spread over the chip), at most grid["tiles"]."""
+        cap = int(self.grid["tiles"])
+        return (ctypes.c_int32 * 4)(*[max(1, min(cap, max(int(c), MIN_LATTICE_GRID)))
+                                      for c in self._lattice_tiles])
+
     def _launch(self):
         P = nat.ptr
         g = self.grid
@@ -400,7 +412,7 @@ class BatchEngine:
             P(self.origin[1]), P(self.origin_ok), P(self.lag), P(self.lx), P(self.lu), P(self.lF),
             P(self.lm), P(self.pstate), P(self.traj), P(self.traj_cap_t), int(g["tiles"]),
             int(g["markers"]), int(g["particles"]), 1, P(self.particle_box), P(self.chunk_particle),
-            P(self.box_cache), self.box_cap, P(self.box_miss), self._mask,
+            P(self.box_cache), self.box_cap, P(self.box_miss), self._mask, self._lattice_grid(),
             nat.stream_handle(self.device),
         )
 

@@ -74,38 +74,55 @@ __device__ inline void with_lattice(int q, Fn&& fn) {
 template <class S, class L>
 __global__ __launch_bounds__(kBlock) void k_batch_fused(
     const CaseDesc* __restrict__ cases, const int* __restrict__ tile_case,
-    const int* __restrict__ tile_local, const int* __restrict__ hdr, BatchDyn dyn,
-    typename S::elem* pool_a, typename S::elem* pool_b, const float* fol_a, const float* fol_b,
-    const uint32_t* __restrict__ fol_tiles) {
+    const int* __restrict__ tile_local, const int* __restrict__ hdr, const int* __restrict__ case_left,
+    const int* __restrict__ case_par, const int* __restrict__ case_done,
+    typename S::elem* __restrict__ pool_a, typename S::elem* __restrict__ pool_b,
+    const float* __restrict__ fol_a, const float* __restrict__ fol_b, const uint32_t* __restrict__ fol_tiles) {
   using E = typename S::elem;
   constexpr int li = L::Q == 13 ? 0 : L::Q == 15 ? 1 : L::Q == 19 ? 2 : 3;
   const int t_begin = li == 0 ? 0 : hdr[H_QEND + li - 1];
   const int t_end = li == 3 ? hdr[H_TILES] : hdr[H_QEND + li];
-  for (int t = t_begin + (int)blockIdx.x; t < t_end; t += gridDim.x) {
-    const int c = tile_case[t];
-    if (c < 0 || dyn.left[c] <= 0) continue;
-    const CaseDesc& cd = cases[c];
-    if (cd.q != L::Q) continue;
-    const uint32_t stamp = case_stamp(dyn, c);  // this step's spreading
+  // consecutive tiles (one case's, whose shifted rows share cache lines) on
+  // one XCD's L2: the hardware deals blocks round-robin over the XCDs
+  const int g = (int)xcd_block(blockIdx.x, gridDim.x);
+  for (int t = t_begin + g; t < t_end; t += gridDim.x) {
+    // the tile's case and place, then everything that depends on the case
+    // only, loaded together before any test: one round trip each (scalar
+    // loads; this kernel writes none of these arrays), not a chain of
+    // tests and loads.  A free tile entry (-1) reads case slot 0 and skips.
+    const int c_raw = tile_case[t];
     const int lt = tile_local[t];
+    const int c = c_raw < 0 ? 0 : c_raw;
+    const CaseDesc& cd = cases[c];
+    const int left = case_left[c];
+    const int par = case_par[c];
+    const uint32_t done = (uint32_t)case_done[c];
+    const int q = cd.q;
+    const int64_t n = cd.p.n;
+    const int64_t f_off = cd.f_off;
+    const int tile_begin = cd.tile_begin;
+    // keeps the compiler from sinking each load behind the test before it
+    // (pinning the pull table as well costs a wave of occupancy: slower)
+    __asm__ volatile("" ::"s"(left), "s"(par), "s"(done), "s"(q), "s"(n), "s"(f_off), "s"(tile_begin));
+    if (c_raw < 0 || left <= 0 || q != L::Q) continue;
     const int64_t node = (int64_t)lt * kBlock + threadIdx.x;
-    if (node >= cd.p.n) continue;
-    const int par = dyn.par[c];
-    E* pa = pool_a + cd.f_off;
-    E* pb = pool_b + cd.f_off;
+    if (node >= n) continue;
+    const uint32_t stamp = done + 1u;  // this step's spreading (case_stamp)
+    E* pa = pool_a + f_off;
+    E* pb = pool_b + f_off;
     const E* src = par ? pb : pa;
     E* dst = par ? pa : pb;
     const float* fa = fol_a + 3 * cd.node_off;
     const float* fb = fol_b + 3 * cd.node_off;
     const float* fo = par ? fb : fa;
     const float* fn = par ? fa : fb;
-    const uint32_t entry = fol_tiles[cd.tile_begin + lt];
+    const uint32_t entry = fol_tiles[tile_begin + lt];
     const bool new_here = entry == stamp;
     const bool old_here = entry == stamp || entry == stamp - 1u;
     FusedIO<S> io{src, dst, fo, fn, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     // the case's dither seed advances with its own step count, as a
     // single-lattice solve's does with its steps
-    const uint32_t mix = mix32(step_seed(cd.p.seed, (uint32_t)dyn.done[c]));
+    const uint32_t mix = mix32(step_seed(cd.p.seed, done));
     fused_step_node<L, S, true>(cd.p, node, io, new_here, old_here, mix);
   }
 }
@@ -390,12 +407,17 @@ int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, in
                       const int32_t* traj_cap, int32_t grid_tiles, int32_t grid_markers,
                       int32_t grid_particles, int32_t with_ibm, const int32_t* particle_box,
                       const int32_t* chunk_particle, float* box_cache, int32_t cache_nodes,
-                      int32_t* miss_list, int32_t lattice_mask, void* stream) {
+                      int32_t* miss_list, int32_t lattice_mask, const int32_t* lattice_grid,
+                      void* stream) {
   if (!cases || !hdr || !case_par || !case_left || !case_done || !case_flag || !tile_case ||
       !tile_local || !pool_a || !pool_b || !fol_a || !fol_b || !fol_tiles)
     return set_error(FUNLBM_EINVAL, "batch_step: NULL array");
   if (grid_tiles <= 0 || grid_markers <= 0 || grid_particles <= 0)
     return set_error(FUNLBM_EINVAL, "batch_step: grid sizes must be positive");
+  if (lattice_grid)
+    for (int i = 0; i < 4; ++i)
+      if (((lattice_mask >> i) & 1) && lattice_grid[i] <= 0)
+        return set_error(FUNLBM_EINVAL, "batch_step: lattice grid %d must be positive", i);
   if (with_ibm && (!marker_case || !marker_particle || !case_marker_begin || !particle_case ||
                    !particle_markers || !owner || !origin_a || !origin_b || !origin_ok || !lag ||
                    !lx || !lu || !lF || !lm || !pstate))
@@ -455,8 +477,11 @@ int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, in
     }
     auto fused = [&](auto l) {
       using L = decltype(l);
-      hipLaunchKernelGGL((k_batch_fused<S, L>), dim3(grid_tiles), dim3(kBlock), 0, s, cd, tile_case,
-                         tile_local, hdr, dyn, (E*)pool_a, (E*)pool_b, (const float*)fol_a,
+      constexpr int li = L::Q == 13 ? 0 : L::Q == 15 ? 1 : L::Q == 19 ? 2 : 3;
+      const int grid = lattice_grid ? lattice_grid[li] : grid_tiles;
+      hipLaunchKernelGGL((k_batch_fused<S, L>), dim3(grid), dim3(kBlock), 0, s, cd, tile_case,
+                         tile_local, hdr, (const int*)case_left, (const int*)case_par,
+                         (const int*)case_done, (E*)pool_a, (E*)pool_b, (const float*)fol_a,
                          (const float*)fol_b, (const uint32_t*)fol_tiles);
       return check_launch("batch_fused");
     };

@@ -368,8 +368,11 @@ int funlbm_case_pack(const funlbm_lattice* lat, int64_t f_offset, int64_t node_o
  * D3Q13, D3Q15, D3Q19, D3Q27) selects the fused kernels launched; a lattice
  * whose bit is clear is not stepped.  Bit 4 runs the per-marker spreading
  * claims, needed while some particle has no box of 0 <= R <= 10 (the
- * others claim through their boxes, as in funlbm_ibm_spread_boxed).  The grid sizes fix the persistent
- * launch shapes. */
+ * others claim through their boxes, as in funlbm_ibm_spread_boxed).  The
+ * grid sizes fix the persistent launch shapes; lattice_grid (host int32[4],
+ * may be NULL: grid_tiles for each) gives the fused kernel of each lattice
+ * its own grid, so it does not sweep the other lattices' tiles.  Every grid
+ * is a grid-stride loop: any positive size gives the same results. */
 int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, int32_t* case_par,
                       int32_t* case_left, int32_t* case_done, int32_t* case_flag,
                       const int32_t* tile_case, const int32_t* tile_local,
@@ -382,7 +385,8 @@ int funlbm_batch_step(int32_t storage, const void* cases, const int32_t* hdr, in
                       const int32_t* traj_cap, int32_t grid_tiles, int32_t grid_markers,
                       int32_t grid_particles, int32_t with_ibm, const int32_t* particle_box,
                       const int32_t* chunk_particle, float* box_cache, int32_t cache_nodes,
-                      int32_t* miss_list, int32_t lattice_mask, void* stream);
+                      int32_t* miss_list, int32_t lattice_mask, const int32_t* lattice_grid,
+                      void* stream);
 
 /* min, mean and max of the reference's f, u and rho after the last fused
  * step, computed from the state without materialising the fields (the
