@@ -194,6 +194,49 @@ def bench_suspension(size=256, n_particles=64, steps=30, warmup=5, storage="f32"
             "ms_per_step": round(ms, 4), "mlups": round(n / (ms * 1e-3) / 1e6, 1), "init_s": round(init_s, 2)}
 
 
+def bench_sphere(steps=200, warmup=20):
+    """The free-fall configuration: D3Q19 64 x 64 x 128 nodes, tau 0.6,
+    walls, a sphere of radius 6 at (32, 32, 100) on Fibonacci markers with
+    Peskin's 4-point kernel, density ratio 1.5, gravity 2e-5 along -z, f32;
+    whole LBMD3 steps, launched eagerly and replayed from HIP graphs."""
+    import torch
+
+    from funlbm_amd.lbm import LBMD3, Config
+
+    out = {"workload": "D3Q19 64x64x128 walls, tau 0.6, sphere r 6 at (32,32,100), Fibonacci markers, "
+                       "Peskin 4-point kernel, rho_p 1.5, gravity 2e-5 -z, f32 state, whole steps"}
+    for graph in (False, True):
+        faces = {f: {"code": "WALL"} for f in ("input", "output", "back", "forward", "bottom", "top")}
+        cfg = Config().from_json({
+            "flow": {"param_type": "D3Q19", "size": [63, 63, 127], "param": {"mu": 0.1 / 3, "gravity": [0, 0, 2e-5]},
+                     "boundary": faces, "ibm_delta": "peskin4"},
+            "particles": [{"type": "sphere", "r": 6, "rou": 1.5, "markers": "fibonacci",
+                           "coord": {"center": [32, 32, 100]}}],
+            "log_every": 0, "graph": graph,
+        })
+        lbm = LBMD3(config=cfg)
+        lbm.init()
+        for k in range(1, warmup + 1):
+            lbm.step = k
+            lbm.run_step()
+        torch.cuda.synchronize()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        for k in range(warmup + 1, warmup + steps + 1):
+            lbm.step = k
+            lbm.run_step()
+        b.record()
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / steps
+        out["graph_ms_per_step" if graph else "eager_ms_per_step"] = round(ms, 4)
+        out["markers"] = lbm.particle_swarm.n_markers
+        out["mlups_graph" if graph else "mlups_eager"] = round(lbm.flow.n / (ms * 1e-3) / 1e6, 1)
+        del lbm
+        torch.cuda.empty_cache()
+    return out
+
+
 def copy_bandwidth(nbytes=2 << 30, reps=10):
     """Measured copy peak: read + write bytes of a plain copy kernel / time."""
     import torch
@@ -617,6 +660,7 @@ def main():
                             "f16_nearest": storage_leg(args.size, 27, "f16", args.steps, args.warmup,
                                                        q27["mlups"], rounding="nearest")}
             out["suspension"] = bench_suspension(size=args.size)
+            out["sphere_free_fall"] = bench_sphere()
         except Exception as e:  # reported, never silent
             out["extra_error"] = repr(e)
         torch.cuda.empty_cache()

@@ -4,6 +4,7 @@ The library is required: importing a compute entry point without it raises
 RuntimeError, there is no CPU fallback.
 """
 
+import contextlib
 import ctypes
 import os
 import threading
@@ -23,6 +24,10 @@ BC_WALL_WITH_SPEED = 1201
 BC_NON_EQUILIBRIUM = 1400
 
 ABI_VERSION = 4
+
+DELTA_COSINE = 0
+DELTA_PESKIN4 = 1
+DELTA_CODES = {"cosine": DELTA_COSINE, "peskin4": DELTA_PESKIN4}
 
 ROUND_NEAREST = 0
 ROUND_STOCHASTIC = 1
@@ -71,6 +76,7 @@ class Lattice(ctypes.Structure):
         ("uw", (c_float * 3) * 6),
         ("rounding", c_int32),
         ("seed", ctypes.c_uint32),
+        ("delta", c_int32),
     ]
 
 
@@ -192,5 +198,21 @@ def state_plane(lat: Lattice) -> int:
     return int(nbytes.value) // (int(lat.q) * esz)
 
 
+_launch_state = threading.local()
+
+
+@contextlib.contextmanager
+def launches_off():
+    """Within the block, call() launches nothing: the host-side bookkeeping
+    of a step that a HIP graph replays runs without its kernels."""
+    _launch_state.off = True
+    try:
+        yield
+    finally:
+        _launch_state.off = False
+
+
 def call(name: str, *args) -> None:
+    if getattr(_launch_state, "off", False):
+        return
     check(getattr(lib(), name)(*args), name)

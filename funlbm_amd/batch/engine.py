@@ -21,6 +21,7 @@ import torch
 
 from .. import _native as nat
 from ..particle import boxes
+from ..particle.markers import markers as marker_points
 from ..particle.markers import surface_markers
 from .alloc import RangeAllocator
 
@@ -42,6 +43,7 @@ class ParticleSpec:
     rou: float = 1.5
     center: tuple = (0.0, 0.0, 0.0)
     angle: tuple = (math.pi / 2, 0.0, 0.0)  # rotation vector, alpha = pi/2 by default
+    markers: str = "cubes"  # "cubes" (the reference's generator) or "fibonacci"
 
 
 @dataclasses.dataclass
@@ -60,6 +62,8 @@ class CaseSpec:
     uzero_mask: int = 0x3F
     # base seed of the stochastic rounding's dither (16-bit states)
     seed: int = 0
+    # immersed-boundary kernel: "cosine" (the reference's) or "peskin4"
+    delta: str = "cosine"
 
     @property
     def n(self):
@@ -78,7 +82,7 @@ def _semi(p: ParticleSpec):
 def prepare_markers(particles) -> None:
     """Generates the markers of every shape among `particles` not seen yet in
     one GPU launch (admission then finds them cached)."""
-    surface_markers([_semi(p) for p in particles])
+    surface_markers([_semi(p) for p in particles if p.markers in (None, "cubes")])
 
 
 def markers_for(p: ParticleSpec):
@@ -86,9 +90,9 @@ def markers_for(p: ParticleSpec):
     following Sphere/Ellipsoid._init (src/funlbm/particle/base/sphere.py:15-53,
     ellipsoid.py:107-143)."""
     a, b, c = _semi(p)
-    key = (p.kind, a, b, c)
+    key = (p.kind, a, b, c, p.markers)
     if key not in _const_cache:
-        lag = surface_markers([(a, b, c)])[0]
+        lag = marker_points(p.markers, a, b, c)
         mass = float(np.float32(4.0 / 3.0 * math.pi * a * b * c))
         if p.kind == "sphere":
             area = float(np.float32(4.0 * math.pi * a * a))
@@ -114,6 +118,7 @@ def lattice_for(spec: CaseSpec, storage: int, dev_scale: float, rounding: int = 
     lat.dev_scale = dev_scale
     lat.rounding = rounding
     lat.seed = int(spec.seed) & 0xFFFFFFFF
+    lat.delta = nat.DELTA_CODES[spec.delta]
     return lat
 
 

@@ -152,6 +152,9 @@ inline int make_params(const funlbm_lattice* lat, KParams* p) {
   if (lat->rounding == FUNLBM_ROUND_STOCHASTIC && !dev)
     return set_error(FUNLBM_EINVAL, "stochastic rounding applies to F16/BF16 storage only");
   p->sr = lat->rounding == FUNLBM_ROUND_STOCHASTIC;
+  if (lat->delta != FUNLBM_DELTA_COSINE && lat->delta != FUNLBM_DELTA_PESKIN4)
+    return set_error(FUNLBM_EINVAL, "unknown immersed-boundary kernel code %d", lat->delta);
+  p->delta = lat->delta;
   p->seed = lat->seed;
   p->seed_mix = mix32(lat->seed);
   p->rho0 = lat->rho0;

@@ -69,9 +69,23 @@ __device__ inline T stencil_sum(const T v[4]) {
 
 // Lane geometry of a marker: node (x0 + i, y, z) of each layer i, which of
 // them lie in the stencil, and their normalised weights.  The cosine kernel
-// (src/funlbm/lbm/lbm3d.py:46-59): phi(r) = (1 + cos(|r pi/2|))/4 per axis,
-// the node's weight (t_x t_y) t_z where all |r_d| <= 2, else 0, r = node - x;
-// the weights of the stencil are normalised to sum 1.
+// (src/funlbm/lbm/lbm3d.py:46-59): phi(r) = (1 + cos(|r pi/2|))/4 per axis
+// (or the Peskin kernel, delta_1d), the node's weight (t_x t_y) t_z where all
+// |r_d| <= 2, else 0, r = node - x; the weights of the stencil are
+// normalised to sum 1.
+// Per-axis kernel value at offset r (|r| <= 2 is tested by the caller):
+// the reference's cosine kernel, or Peskin's 4-point kernel
+// (C. S. Peskin, Acta Numerica 11 (2002) 479-517, eq. 6.27), whose values at
+// the four nodes around a point sum to 1 and whose first moment vanishes.
+__device__ inline float delta_1d(const KParams& p, float r) {
+  if (p.delta == FUNLBM_DELTA_PESKIN4) {
+    const float a = fabsf(r);
+    if (a <= 1.0f) return (3.0f - 2.0f * a + sqrtf(1.0f + 4.0f * a - 4.0f * a * a)) / 8.0f;
+    return (5.0f - 2.0f * a - sqrtf(fmaxf(-7.0f + 12.0f * a - 4.0f * a * a, 0.0f))) / 8.0f;
+  }
+  return (1.0f + cosf(fabsf(r * kPiF / 2.0f))) / 4.0f;
+}
+
 struct LaneW {
   int x0, y, z;
   unsigned valid;  // bit i: node i inside the stencil
@@ -90,8 +104,8 @@ __device__ inline LaneW lane_weight(const KParams& p, const float x[3], int sub)
   // unnormalised weight per node: (t_x t_y) t_z, t_y and t_z shared by the layers
   const float ry = (float)lw.y - x[1], rz = (float)lw.z - x[2];
   const bool inyz = fabsf(ry) <= 2.0f && fabsf(rz) <= 2.0f;
-  const float ty = (1.0f + cosf(fabsf(ry * kPiF / 2.0f))) / 4.0f;
-  const float tz = (1.0f + cosf(fabsf(rz * kPiF / 2.0f))) / 4.0f;
+  const float ty = delta_1d(p, ry);
+  const float tz = delta_1d(p, rz);
   lw.valid = 0;
   float w[4];
 #pragma unroll
@@ -99,7 +113,7 @@ __device__ inline LaneW lane_weight(const KParams& p, const float x[3], int sub)
     const bool v = vyz && lw.x0 + i < s.hi[0];
     lw.valid |= (unsigned)v << i;
     const float rx = (float)(lw.x0 + i) - x[0];
-    const float tx = (1.0f + cosf(fabsf(rx * kPiF / 2.0f))) / 4.0f;
+    const float tx = delta_1d(p, rx);
     w[i] = (v && inyz && fabsf(rx) <= 2.0f) ? (tx * ty) * tz : 0.0f;
   }
   const float ws = stencil_sum(w);

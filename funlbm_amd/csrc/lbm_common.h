@@ -264,6 +264,7 @@ struct KParams {
   int has_neq;     // a NON_EQUILIBRIUM input/output face (explicit path only)
   int has_body;    // body_force != 0
   int sr;          // stochastic rounding of a 16-bit state's stores
+  int delta;       // FUNLBM_DELTA_*: the immersed-boundary kernel
   uint32_t seed;   // the lattice's dither seed (the base, for a batched case)
   uint32_t seed_mix;  // dither_mix(seed): this step's mix (single-lattice calls)
   float tau;       // float32 tau as the reference's field

@@ -114,6 +114,10 @@ class FlowConfig(BaseConfig):
                    (src/funlbm/particle/base/base.py:138-142).  With this sign a
                    particle heavier than the fluid accelerates against g (the
                    reference's free-fall example sets gl < 0).
+      ibm_delta    "cosine" (default, the reference's (1 + cos(pi r/2))/4,
+                   src/funlbm/lbm/lbm3d.py:46-64) or "peskin4" (Peskin's
+                   4-point kernel): the per-axis kernel of the
+                   immersed-boundary interpolation and spreading weights
     """
 
     def __init__(self, *args, **kwargs):
@@ -131,6 +135,7 @@ class FlowConfig(BaseConfig):
         self.uzero_faces = None
         self.body_force = None
         self.gravity = None
+        self.ibm_delta: str = "cosine"
         super().__init__(*args, **kwargs)
 
     def _from_json(self, config_json: dict, *args, **kwargs):
@@ -148,6 +153,7 @@ class FlowConfig(BaseConfig):
         self.uzero_faces = deep_get(config_json, "uzero_faces")
         self.body_force = deep_get(config_json, "body_force")
         self.gravity = deep_get(self.param, "gravity") or deep_get(config_json, "gravity")
+        self.ibm_delta = str(deep_get(config_json, "ibm_delta") or self.ibm_delta)
 
 
 class Flow:

@@ -75,6 +75,10 @@ class FlowD3(Flow):
         self.uzero_mask = _uzero_mask(getattr(config, "uzero_faces", None))
         bf = getattr(config, "body_force", None)
         self.body_force = None if bf is None else [float(v) for v in bf]
+        delta = getattr(config, "ibm_delta", None) or "cosine"
+        if delta not in nat.DELTA_CODES:
+            raise ValueError(f"unknown ibm_delta {delta!r}; expected one of {list(nat.DELTA_CODES)}")
+        self.ibm_delta = delta
 
     # ------------------------------------------------------------------
     # lattice descriptors
@@ -98,6 +102,7 @@ class FlowD3(Flow):
         if storage_code == nat.STORAGE_CODES[self.storage]:
             lat.rounding = nat.ROUNDING_CODES[self.rounding]
         lat.seed = self.seed
+        lat.delta = nat.DELTA_CODES[self.ibm_delta]
         if self.body_force is not None:
             for d in range(3):
                 lat.body_force[d] = self.body_force[d]

@@ -24,6 +24,12 @@ class Config(BaseConfig):
                  file.cache_dir/checkpoint (default 0: none; the reference
                  writes every file.per_steps = 100 steps, 1.3 GB each for a
                  256^3 D3Q19 lattice)
+      output_every  write the reference's save_bak file set (Tecplot and VTK
+                 files of u, rho, p and the markers, under file.cache_dir)
+                 every this many steps (default 0: never, as the reference,
+                 whose save_bak is not called)
+      graph      replay the fused particle step from HIP graphs once it is
+                 periodic (default false: no faster on ROCm 7.2; see LBMD3)
       async_log  run() copies its per-step records back without waiting for
                  the GPU and emits them up to 8 steps late, in order
                  (default true; false waits for each record, as the
@@ -47,6 +53,8 @@ class Config(BaseConfig):
         self.checkpoint_every: int = 0
         self.db_path = None
         self.async_log: bool = True
+        self.output_every: int = 0
+        self.graph: bool = False
 
     def _from_json(self, config_json: dict, *args, **kwargs) -> "Config":
         from ..particle import ParticleConfig
@@ -68,6 +76,8 @@ class Config(BaseConfig):
         self.db_path = deep_get(config_json, "db_path") or self.db_path
         al = deep_get(config_json, "async_log")
         self.async_log = True if al is None else bool(al)
+        self.output_every = int(deep_get(config_json, "output_every") or 0)
+        self.graph = bool(deep_get(config_json, "graph") or False)
         return self
 
 
@@ -173,6 +183,9 @@ class LBMBase(Worker):
             self._handle_immersed_boundary()
             self._update_particles()
         self.save(self.step)
+        every = self.config.output_every
+        if every and self.step % every == 0:
+            self.save_bak(self.step, force=True)
 
     def _compute_flow(self) -> None:
         """src/funlbm/lbm/base.py:159-163."""
@@ -226,6 +239,16 @@ class LBMBase(Worker):
         path = f"{self.config.file_config.checkpoint_path}/checkpoint_{str(step).zfill(10)}.npz"
         return self.dump_checkpoint(path)
 
+    def save_bak(self, step=10, force=False, *args, **kwargs):
+        """Tecplot and VTK files of the flow and the markers at a step
+        divisible by file.per_steps (src/funlbm/lbm/lbm3d.py:169-244), or
+        at any step with force; returns the paths written (funlbm_amd.lbm.output)."""
+        if not force and step % self.config.file_config.per_steps > 0:
+            return []
+        from .output import save_fields
+
+        return save_fields(self, step)
+
     def dump_checkpoint(self, path: str) -> str:
         """src/funlbm/lbm/base.py:205-210."""
         from .checkpoint import dump
@@ -242,3 +265,5 @@ class LBMBase(Worker):
             logger.error("load failed: checkpoint %r not found", checkpoint_path)
             return
         load(self, checkpoint_path)
+        if hasattr(self, "reset_graphs"):
+            self.reset_graphs()

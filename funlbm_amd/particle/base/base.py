@@ -16,16 +16,21 @@ from ..coord import CoordConfig, Coordinate
 
 
 class ParticleConfig(BaseConfig):
-    """src/funlbm/particle/base/base.py:11-19; type defaults to "ellipsoid"."""
+    """src/funlbm/particle/base/base.py:11-19; type defaults to "ellipsoid".
+    Extension key: "markers", "cubes" (default: the reference's
+    cube-intersection markers) or "fibonacci" (a golden-angle lattice at
+    about dx spacing)."""
 
     def __init__(self, *args, **kwargs):
         super().__init__(*args, **kwargs)
         self.coord_config: CoordConfig = CoordConfig()
         self.type = "ellipsoid"
+        self.markers = "cubes"
 
     def _from_json(self, config_json: dict, *args, **kwargs):
         self.coord_config.from_json(deep_get(config_json, "coord") or {})
         self.type = deep_get(config_json, "type") or self.type
+        self.markers = deep_get(config_json, "markers") or self.markers
 
 
 class Particle:

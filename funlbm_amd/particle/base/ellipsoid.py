@@ -4,7 +4,7 @@ import math
 
 import numpy as np
 
-from ..markers import surface_markers
+from ..markers import markers
 from .base import Particle
 
 
@@ -19,7 +19,7 @@ class Ellipsoid(Particle):
 
     def _init(self, dx=1, *args, **kwargs):
         a, b, c = self.ra, self.rb, self.rc
-        self._lagrange = surface_markers([(a, b, c)], dx, device=self.device)[0]
+        self._lagrange = markers(self.config.markers, a, b, c, dx, device=self.device)
         self.mass = float(np.float32(4.0 / 3.0 * math.pi * a * b * c))
         # the reference's area expression 4/3 pi (abc)^(2/3)
         self.area = float(np.float32(4.0 / 3.0 * math.pi * math.pow(a * b * c, 2.0 / 3.0)))

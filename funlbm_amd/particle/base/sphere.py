@@ -4,7 +4,7 @@ import math
 
 import numpy as np
 
-from ..markers import surface_markers
+from ..markers import markers
 from .base import Particle
 
 
@@ -15,7 +15,7 @@ class Sphere(Particle):
 
     def _init(self, dx=1, *args, **kwargs):
         r = self.r
-        self._lagrange = surface_markers([(r, r, r)], dx, device=self.device)[0]
+        self._lagrange = markers(self.config.markers, r, r, r, dx, device=self.device)
         self.mass = float(np.float32(4.0 / 3.0 * math.pi * r * r * r))
         self.area = float(np.float32(4.0 * math.pi * r * r))
         # float32(r^2 * float32(V) / 5) per axis, as the reference's numpy expression

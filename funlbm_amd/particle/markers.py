@@ -83,3 +83,42 @@ def _generate(todo, dx, dev):
         arr = np.ascontiguousarray(host[i, : int(cnt[i])])
         arr.setflags(write=False)
         _cache[k] = arr
+
+
+def fibonacci_markers(a: float, b: float, c: float, dx: float = 1.0) -> np.ndarray:
+    """Body-frame markers on a Fibonacci (golden-angle) lattice, a
+    non-default option beside the reference's cube-intersection markers:
+    N = round(A / dx^2) points for surface area A (exact for a sphere,
+    Thomsen's approximation for an ellipsoid), one per dx^2 of surface;
+    nearest neighbours then lie about 0.95 dx apart.  Point i of the
+    unit sphere is (rho cos t, rho sin t, z) with z = 1 - (2i + 1)/N,
+    rho = sqrt(1 - z^2), t = i pi (3 - sqrt 5) (equal-area bands, golden
+    angle); an ellipsoid scales it by (a, b, c).  float32 [N, 3], read-only."""
+    a, b, c, dx = (float(v) for v in (a, b, c, dx))
+    if not all(v > 0.0 and math.isfinite(v) for v in (a, b, c, dx)):
+        raise ValueError(f"semi-axes and spacing must be positive, got {(a, b, c)}, {dx}")
+    key = ("fibonacci", a, b, c, dx)
+    if key not in _fib_cache:
+        area = 4.0 * math.pi * a * a if a == b == c else _area_estimate(a, b, c)
+        n = max(4, int(round(area / (dx * dx))))
+        i = np.arange(n, dtype=np.float64)
+        z = 1.0 - (2.0 * i + 1.0) / n
+        rho = np.sqrt(np.maximum(1.0 - z * z, 0.0))
+        t = i * (math.pi * (3.0 - math.sqrt(5.0)))
+        pts = np.stack([a * rho * np.cos(t), b * rho * np.sin(t), c * z], axis=1).astype(np.float32)
+        pts.setflags(write=False)
+        _fib_cache[key] = pts
+    return _fib_cache[key]
+
+
+_fib_cache: Dict[tuple, np.ndarray] = {}
+
+
+def markers(kind: str, a: float, b: float, c: float, dx: float = 1.0, device=None) -> np.ndarray:
+    """Markers of one shape by method: "cubes" (the reference's
+    cube-intersection generator, on the GPU) or "fibonacci"."""
+    if kind == "fibonacci":
+        return fibonacci_markers(a, b, c, dx)
+    if kind not in (None, "cubes"):
+        raise ValueError(f"unknown marker method {kind!r}; expected 'cubes' or 'fibonacci'")
+    return surface_markers([(a, b, c)], dx, device=device)[0]

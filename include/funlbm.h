@@ -61,6 +61,14 @@ extern "C" {
                                      unbiased, so increments below half an ulp of the
                                      stored deviation accumulate instead of being lost */
 
+/* One-dimensional kernel of the immersed-boundary weights (per axis, the
+ * 3-D weight is the product, normalised over the stencil). */
+#define FUNLBM_DELTA_COSINE 0  /* (1 + cos(pi r/2))/4 for |r| <= 2: the reference's
+                                  (src/funlbm/lbm/lbm3d.py:46-64) */
+#define FUNLBM_DELTA_PESKIN4 1 /* Peskin's 4-point kernel: (3 - 2|r| + sqrt(1 + 4|r| - 4r^2))/8
+                                  for |r| <= 1, (5 - 2|r| - sqrt(-7 + 12|r| - 4r^2))/8 for
+                                  1 <= |r| <= 2 (extension; no reference counterpart) */
+
 typedef struct funlbm_lattice {
   int32_t q;          /* 13, 15, 19 or 27 */
   int32_t nx, ny, nz; /* node counts */
@@ -78,6 +86,8 @@ typedef struct funlbm_lattice {
                          (a solve passes base + steps_done * 0x9E3779B9, mod 2^32, so
                          every step draws new dithers); funlbm_case_pack takes the
                          base and the batched step adds its own step count */
+  int32_t delta;      /* FUNLBM_DELTA_*: the interpolation/spreading kernel of the
+                         immersed-boundary entry points */
 } funlbm_lattice;
 
 /* ABI version (FUNLBM_ABI_VERSION).  No reference counterpart. */

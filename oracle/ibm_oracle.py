@@ -9,7 +9,8 @@ Restates, with per-marker Python loops like the reference:
                        closed-form root, vectorised, pinned against it)
   particle constants   src/funlbm/particle/base/sphere.py:15-53,
                        ellipsoid.py:107-143, base.py:108-118
-  stencil / weights    src/funlbm/lbm/lbm3d.py:32-64
+  stencil / weights    src/funlbm/lbm/lbm3d.py:32-64 (and Peskin's 4-point
+                       kernel as the build's option)
   flow_to_lagrange     src/funlbm/lbm/lbm3d.py:66-100
   particle_to_wall     src/funlbm/lbm/lbm3d.py:121-167
   update_from_lar      src/funlbm/particle/base/base.py:120-151
@@ -220,11 +221,26 @@ def region_bounds(lx, size):
     return rl.astype(np.int32), rr.astype(np.int32)
 
 
-def weight_function(r):
-    """src/funlbm/lbm/lbm3d.py:46-64 (float32)."""
+def peskin4(r):
+    """Peskin's 4-point kernel (Acta Numerica 11 (2002), eq. 6.27), float32,
+    for |r| <= 2 (the caller masks the rest).  Not a reference function: the
+    option the build adds beside the reference's cosine kernel."""
+    a = np.abs(np.asarray(r, dtype=np.float32))
+    one, two, three, four, five, seven, eight, twelve = (np.float32(v) for v in (1, 2, 3, 4, 5, 7, 8, 12))
+    inner = (three - two * a + np.sqrt(np.maximum(one + four * a - four * a * a, 0))) / eight
+    outer = (five - two * a - np.sqrt(np.maximum(-seven + twelve * a - four * a * a, 0))) / eight
+    return np.where(a <= one, inner, outer).astype(np.float32)
+
+
+def weight_function(r, delta="cosine"):
+    """src/funlbm/lbm/lbm3d.py:46-64 (float32); delta="peskin4" swaps the
+    per-axis cosine kernel for peskin4."""
     mask = np.all(np.abs(r) <= 2, axis=-1, keepdims=True)
-    tmp = np.where(mask, (np.float32(1) + np.cos(np.abs(r * np.float32(np.pi) / np.float32(2)))) / np.float32(4),
-                   np.float32(0)).astype(np.float32)
+    if delta == "peskin4":
+        phi = peskin4(r)
+    else:
+        phi = (np.float32(1) + np.cos(np.abs(r * np.float32(np.pi) / np.float32(2)))) / np.float32(4)
+    tmp = np.where(mask, phi, np.float32(0)).astype(np.float32)
     w = (tmp[..., 0] * tmp[..., 1] * tmp[..., 2])[..., None]
     s = np.sum(w)
     if s > 0:
@@ -237,7 +253,7 @@ def _grid(rl, rr):
     return np.stack([gx, gy, gz], axis=-1)
 
 
-def flow_to_lagrange(st: lbo.FlowState, particles, size, dt, step):
+def flow_to_lagrange(st: lbo.FlowState, particles, size, dt, step, delta="cosine"):
     """src/funlbm/lbm/lbm3d.py:66-100; returns False when a weight sum fails
     the check after step 100."""
     ok = True
@@ -248,7 +264,7 @@ def flow_to_lagrange(st: lbo.FlowState, particles, size, dt, step):
         for m, lar in enumerate(p.lx):
             sl = (slice(rl[m, 0], rr[m, 0]), slice(rl[m, 1], rr[m, 1]), slice(rl[m, 2], rr[m, 2]))
             x = _grid(rl[m], rr[m]).astype(np.float32)
-            w = weight_function(x - lar)
+            w = weight_function(x - lar, delta)
             if abs(np.sum(w) - 1) > 1e-3 and step > 100:
                 ok = False
             lu[m, :] = np.sum(st.u[sl] * w)
@@ -276,7 +292,7 @@ def particle_to_wall(particles, size, dx=1.0):
             p.lF[ext[j]] = rows[j]
 
 
-def lagrange_to_flow(st: lbo.FlowState, particles, size):
+def lagrange_to_flow(st: lbo.FlowState, particles, size, delta="cosine"):
     """src/funlbm/lbm/lbm3d.py:102-119: FOL := 0, then overwrite per marker."""
     st.FOL = np.zeros_like(st.FOL)
     for p in particles:
@@ -284,20 +300,20 @@ def lagrange_to_flow(st: lbo.FlowState, particles, size):
         for m, lar in enumerate(p.lx):
             sl = (slice(rl[m, 0], rr[m, 0]), slice(rl[m, 1], rr[m, 1]), slice(rl[m, 2], rr[m, 2]))
             x = _grid(rl[m], rr[m]).astype(np.float32)
-            w = weight_function(x - lar)
+            w = weight_function(x - lar, delta)
             st.FOL[sl] = (w * p.lF[m, :] * p.lm[m]).astype(st.dt)
 
 
-def run_step(st: lbo.FlowState, particles, size, dt=1.0, dx=1.0, gl=0.0, step=1):
+def run_step(st: lbo.FlowState, particles, size, dt=1.0, dx=1.0, gl=0.0, step=1, delta="cosine"):
     """LBMBase.run_step (src/funlbm/lbm/base.py:146-179); returns run_status."""
     lbo.cul_equ(st)
     lbo.f_stream(st)
     lbo.update_u_rou(st)
-    ok = flow_to_lagrange(st, particles, size, dt, step)
+    ok = flow_to_lagrange(st, particles, size, dt, step, delta)
     particle_to_wall(particles, size, dx)
     for p in particles:
         p.update_from_lar(dt, gl)
-    lagrange_to_flow(st, particles, size)
+    lagrange_to_flow(st, particles, size, delta)
     lbo.cul_equ2(st)
     lbo.update_u_rou(st)
     for p in particles:

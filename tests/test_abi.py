@@ -37,8 +37,8 @@ def test_bindings_cover_header():
 def test_abi_version_and_struct_size():
     lib = nat.lib()
     assert lib.funlbm_abi_version() == 4
-    # 5 ints + 6 ints + 1 int + 3 floats + 3 floats + 18 floats + rounding + seed
-    assert ctypes.sizeof(nat.Lattice) == 4 * (5 + 6 + 1 + 3 + 3 + 18 + 2)
+    # 5 ints + 6 ints + 1 int + 3 floats + 3 floats + 18 floats + rounding + seed + delta
+    assert ctypes.sizeof(nat.Lattice) == 4 * (5 + 6 + 1 + 3 + 3 + 18 + 3)
 
 
 def test_rounding_modes_are_checked_on_the_host():

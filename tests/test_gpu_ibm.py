@@ -315,3 +315,101 @@ def test_sphere_matches_oracle_1000_steps():
     assert ef < 1e-5, ef
     assert ec < 1e-4, ec
     assert travel > 1e-2  # it moved
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_graph_replayed_steps_equal_eager_steps(storage):
+    """Particle steps replayed from HIP graphs (one per parity of the
+    ping-pong pairs and per state of the weight-sum check) give the bits of
+    eager steps, across step 100 where the check arms; f16 here rounds to
+    nearest (stochastic rounding steps eagerly)."""
+    runs = []
+    for graph in (True, False):
+        lbm = make_case(particles=SPHERE, extra_flow={"storage": storage, "rounding": "nearest"})
+        lbm.config.graph = graph
+        for s in range(1, 111):
+            lbm.step = s
+            lbm.run_step()
+        torch.cuda.synchronize()
+        runs.append(lbm)
+    a, b = runs
+    assert len(a._graphs) == 4 and len(b._graphs) == 0
+    assert torch.equal(a.particle_swarm.pstate, b.particle_swarm.pstate)
+    assert torch.equal(a.particle_swarm.lx, b.particle_swarm.lx)
+    assert torch.equal(a.flow.f, b.flow.f)
+    assert torch.equal(a.flow.FOL, b.flow.FOL)
+
+
+def test_graph_steps_with_logged_steps_in_run():
+    """run() with per-step statistics every 3 steps: the logged steps run
+    eagerly between graph replays, and the records and the final state equal
+    those of an eager run."""
+    runs = []
+    for graph in (True, False):
+        lbm = make_case(particles=SPHERE)
+        lbm.config.graph = graph
+        lbm.config.max_step = 20
+        lbm.config.log_every = 3
+        lbm.run()
+        runs.append(lbm)
+    a, b = runs
+    assert len(a._graphs) > 0
+    assert [r["flow"] for r in a.history] == [r["flow"] for r in b.history]
+    assert torch.equal(a.particle_swarm.pstate, b.particle_swarm.pstate)
+    assert torch.equal(a.flow.f, b.flow.f)
+
+
+def test_fibonacci_markers_and_peskin_kernel_match_oracle():
+    """The free-fall configuration with the build's two options: a sphere of
+    radius 6 on Fibonacci markers and Peskin's 4-point kernel (64 x 64 x 128
+    nodes, tau 0.6, walls, density ratio 1.5, gravity along -z), 30 fused
+    steps against the oracle with the same markers and kernel."""
+    size = (63, 63, 127)
+    sphere = [{"type": "sphere", "r": 6, "rou": 1.5, "markers": "fibonacci",
+               "coord": {"center": [32, 32, 100]}}]
+    g = [0.0, 0.0, 2e-5]
+    lbm = make_case(size=size, particles=sphere, extra_flow={"ibm_delta": "peskin4", "gravity": g})
+    p = lbm.particle_swarm.particles[0]
+    assert p.n_markers == round(4 * np.pi * 36)
+    st, parts = oracle_case(lbm, size=size, gravity=g)
+    for s in range(1, 31):
+        lbm.step = s
+        lbm.run_step()
+        ibo.run_step(st, parts, list(size), dt=1.0, gl=0.0, step=s, delta="peskin4")
+    o = parts[0]
+    assert np.allclose(p.cx.cpu().numpy(), o.cx, rtol=0, atol=1e-6)
+    assert np.max(np.abs(p.lx.cpu().numpy() - o.lx)) < 1e-5
+    assert rel_linf(lbm.flow.f.cpu().numpy(), st.f) < 1e-5
+    assert rel_linf(lbm.flow.FOL.cpu().numpy(), st.FOL) < 1e-4
+    # the cosine kernel gives different forces: the option is in effect
+    ref = make_case(size=size, particles=sphere, extra_flow={"gravity": g})
+    for s in range(1, 31):
+        ref.step = s
+        ref.run_step()
+    assert not torch.equal(ref.flow.FOL, lbm.flow.FOL)
+
+
+def test_output_files_of_a_run(tmp_path):
+    """output_every writes the save_bak file set; the VTK flow file holds the
+    solver's u and rho, the marker file its lx."""
+    from tests.test_output_format import _read_legacy_vtk
+
+    lbm = make_case(particles=SPHERE)
+    lbm.config.output_every = 5
+    lbm.config.file_config.cache_dir = str(tmp_path)
+    for s in range(1, 11):
+        lbm.step = s
+        lbm.run_step()
+    names = sorted(p.name for p in (tmp_path / "vtk").iterdir())
+    assert names == ["flow_0000000005.vtk", "flow_0000000010.vtk", "particle_000_0000000005.vtk",
+                     "particle_000_0000000010.vtk"]
+    assert len(list((tmp_path / "tecplot").iterdir())) == 6
+    _, meta, blocks = _read_legacy_vtk(str(tmp_path / "vtk" / "flow_0000000010.vtk"))
+    u = lbm.flow.u.cpu().numpy()
+    X, Y, Z = u.shape[:3]
+    assert meta["DIMENSIONS"] == [str(X), str(Y), str(Z)]
+    assert np.array_equal(blocks["u"].reshape(Z, Y, X, 3).transpose(2, 1, 0, 3), u)
+    rho = lbm.flow.rou.cpu().numpy()[..., 0]
+    assert np.array_equal(blocks["rho"].reshape(Z, Y, X).transpose(2, 1, 0), rho)
+    _, _, pb = _read_legacy_vtk(str(tmp_path / "vtk" / "particle_000_0000000010.vtk"))
+    assert np.array_equal(pb["POINTS"], lbm.particle_swarm.lx.cpu().numpy())

@@ -305,3 +305,40 @@ def test_on_node_interpolation_of_a_linear_field():
     p.cw = np.zeros(3)
     ibo.flow_to_lagrange(st, [p], SIZE, 1.0, 1)
     assert abs(float(p.lF[0, 0]) - (-2.0 * UNIFORM_RHO * 5.0)) < 1e-5
+
+
+# ---------------------------------------------------------------- Peskin kernel
+def test_peskin_kernel_moment_conditions():
+    """Peskin's 4-point kernel: over the four nodes around any point its
+    values sum to 1 (even and odd nodes to 1/2 each), its first moment
+    vanishes and its squares sum to 3/8 (Acta Numerica 11 (2002) 479-517,
+    eqs. 6.8-6.12); phi(0) = 1/2, phi(1) = 1/4, phi(2) = 0, continuous at 1."""
+    x = np.linspace(0.0, 1.0, 101, dtype=np.float64)[:, None]
+    j = np.array([-1, 0, 1, 2], dtype=np.float64)[None, :]
+    r = (j - x).astype(np.float32)
+    phi = ibo.peskin4(r).astype(np.float64)
+    assert np.allclose(phi.sum(1), 1.0, atol=2e-7)
+    assert np.allclose(phi[:, ::2].sum(1), 0.5, atol=2e-7)
+    assert np.allclose((phi * (j - x)).sum(1), 0.0, atol=5e-7)
+    assert np.allclose((phi * phi).sum(1), 3.0 / 8.0, atol=5e-7)
+    v = ibo.peskin4(np.array([0.0, 1.0, 2.0, 1.0 - 1e-6, 1.0 + 1e-6], np.float32))
+    assert np.allclose(v[:3], [0.5, 0.25, 0.0], atol=1e-7)
+    assert abs(float(v[3]) - float(v[4])) < 1e-5
+
+
+def test_peskin_weights_interpolate_linear_fields_exactly():
+    """With Peskin weights an interior marker interpolates u_x = x exactly
+    (first moment condition), unlike the cosine kernel off the nodes."""
+    st, p = _uniform_case()
+    st.u = np.zeros_like(st.u)
+    st.u[..., 0] = np.arange(st.u.shape[0], dtype=np.float32)[:, None, None]
+    p.lx = np.array([[5.3, 6.6, 7.1], [8.85, 4.2, 3.5], [3.5, 9.9, 6.25], [10.05, 5.5, 4.75]], np.float32)
+    p.cu = np.zeros(3)
+    p.cw = np.zeros(3)
+    ibo.flow_to_lagrange(st, [p], SIZE, 1.0, 1, delta="peskin4")
+    s_u = -p.lF[:, 0].astype(np.float64) / (2.0 * UNIFORM_RHO)
+    assert np.allclose(s_u, p.lx[:, 0], atol=2e-5)
+    p.lu = np.zeros_like(p.lu)
+    ibo.flow_to_lagrange(st, [p], SIZE, 1.0, 1)  # the cosine kernel is not exact
+    s_cos = -p.lF[:, 0].astype(np.float64) / (2.0 * UNIFORM_RHO)
+    assert np.max(np.abs(s_cos - p.lx[:, 0])) > 1e-3
