@@ -444,7 +444,7 @@ def bench_batch(n_cases=256, steps=3000, seed=0, storage="f16", tick=50, baselin
 
 
 def serve_rate(rate, duration, seed=0, steps=(1000, 20000), budget=200e9, storage="f16", tick=50,
-               drain_s=6.0, pool_nodes_cap=1 << 28):
+               drain_s=6.0, pool_nodes_cap=1 << 28, max_active_nodes=None):
     """Poisson stream of settling requests at `rate` req/s for `duration` s,
     served with continuous batching; the run is cut `drain_s` after the last
     arrival (requests still queued or running then are the backlog)."""
@@ -461,14 +461,17 @@ def serve_rate(rate, duration, seed=0, steps=(1000, 20000), budget=200e9, storag
     markers = sum(len(markers_for(p)[0]) for _, s in reqs for p in s.particles)
     eng = BatchEngine(storage=storage, max_nodes=pool_nodes, max_markers=markers + 64,
                       max_particles=4 * len(reqs) + 4, max_cases=len(reqs) + 1, max_q=27, traj_cap=tick)
-    srv = Server(eng, hbm_budget=budget, tick_steps=tick)
+    srv = Server(eng, hbm_budget=budget, tick_steps=tick, max_active_nodes=max_active_nodes)
     stats = srv.run([Request(rid=i, spec=s, arrival=t) for i, (t, s) in enumerate(reqs)],
                     max_seconds=duration + drain_s)
+    # lattice updates done by the GPU in the run, finished or not
+    stats["glups_stepped"] = sum(r.spec.n * (r.spec.steps - r.left) for r in srv.done + list(srv.running.values())) \
+        / stats["wall_s"] / 1e9
     del srv, eng
     torch.cuda.empty_cache()
     out = {"rate": rate}
-    for k in ("submitted", "completed", "backlog", "solves_per_s", "mlups", "p50_latency_s", "p99_latency_s",
-              "wall_s", "wait_s", "ticks"):
+    for k in ("submitted", "completed", "backlog", "solves_per_s", "mlups", "glups_stepped", "p50_latency_s",
+              "p99_latency_s", "wall_s", "wait_s", "ticks"):
         v = stats.get(k)
         out[k] = round(v, 3) if isinstance(v, float) else v
     return out
@@ -510,13 +513,15 @@ def serve_error_check(seed=0, steps=(1000, 20000), storage="f16"):
     }
 
 
-def bench_serve(rates, duration, steps=(1000, 20000), budget=200e9, storage="f16", tick=50, drain_s=6.0):
-    sweep = [serve_rate(r, duration, steps=steps, budget=budget, storage=storage, tick=tick, drain_s=drain_s)
-             for r in rates]
+def bench_serve(rates, duration, steps=(1000, 20000), budget=200e9, storage="f16", tick=50, drain_s=6.0,
+                max_active_nodes=1 << 25):
+    sweep = [serve_rate(r, duration, steps=steps, budget=budget, storage=storage, tick=tick, drain_s=drain_s,
+                        max_active_nodes=max_active_nodes) for r in rates]
     return {
         "workload": f"Poisson arrivals for {duration} s per rate (run cut {drain_s} s after the last arrival), "
                     f"sides log-uniform 32-128, D3Q19/27, 1-4 spheres or ellipsoids, steps {steps[0]}-{steps[1]}, "
-                    f"{storage}-deviation storage, {budget / 1e9:.0f} GB budget, ticks of {tick} steps, "
+                    f"{storage}-deviation storage, {budget / 1e9:.0f} GB budget, at most {max_active_nodes} "
+                    f"nodes stepping at once (first come, first served), ticks of {tick} steps, "
                     f"two ticks in flight",
         "sweep": sweep,
         "error_check": serve_error_check(steps=steps, storage=storage),

@@ -16,7 +16,9 @@ waits for the oldest tick's event only, streams out the trajectory rows of
 the steps taken in that tick, retires the requests that finished in it and
 admits waiting ones (admission enqueues its copies asynchronously too).  The
 host knows how many steps each admitted case takes per tick, so it launches
-a tick only while some case still has steps left.
+a tick only while some case still has steps left.  Admission is first come,
+first served, under the HBM budget and, optionally, a cap on the lattice
+nodes stepping at once.
 
 No reference counterpart: the reference runs one case per process
 (src/funlbm/server/script.py:10-24).
@@ -77,7 +79,7 @@ class _Snapshot:
 
 class Server:
     def __init__(self, engine: BatchEngine, hbm_budget: float, tick_steps: int = 50,
-                 stream_trajectories: bool = True, depth: int = 2):
+                 stream_trajectories: bool = True, depth: int = 2, max_active_nodes: Optional[int] = None):
         if engine.traj_cap < tick_steps:
             raise ValueError("the trajectory ring must hold one tick of steps")
         if depth < 1:
@@ -87,6 +89,11 @@ class Server:
         self.tick_steps = int(tick_steps)
         self.stream = stream_trajectories
         self.depth = int(depth)
+        # lattice nodes admitted at once (None: the HBM budget alone limits
+        # admission): enough to keep the GPU busy, few enough that requests
+        # finish in arrival order instead of all sharing the GPU
+        self.max_active_nodes = max_active_nodes
+        self.active_nodes = 0
         self.in_use = 0.0
         self.queue: deque = deque()
         self.running: Dict[int, Request] = {}
@@ -109,12 +116,16 @@ class Server:
             need = self.footprint(req.spec)
             if self.in_use + need > self.budget or not self.engine.fits(req.spec):
                 break
+            if (self.max_active_nodes is not None and self.running
+                    and self.active_nodes + req.spec.n > self.max_active_nodes):
+                break
             self.queue.popleft()
             req.cid = self.engine.add(req.spec)
             req.admitted = now
             req.first_tick = self._ticks
             req.left = int(req.spec.steps)
             self.in_use += need
+            self.active_nodes += req.spec.n
             self.running[req.cid] = req
 
     def _launch(self):
@@ -160,6 +171,7 @@ class Server:
             req.result = ps[s.p_off : s.p_off + s.p_count].copy()
             req.finished = now
             self.in_use -= self.footprint(req.spec)
+            self.active_nodes -= req.spec.n
             eng.remove(cid)
             self.done.append(req)
         self._free.append(snap)
