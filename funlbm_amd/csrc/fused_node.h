@@ -214,6 +214,48 @@ __device__ __forceinline__ void collide_store_dev(const KParams& p, int64_t node
 // are written out here as in collide_store_dev: with the call, the compiler
 // merged the multi-lattice kernel's pull paths differently and that kernel
 // ran 40 % slower on the MI355X.
+// Store of a full wave's 16-bit populations through LDS: each lane writes
+// its Q values into the wave's staging area (direction-major, 64 nodes per
+// direction), then the wave stores 16-byte chunks (8 consecutive nodes of one
+// direction) with one global_store_dwordx4 per lane: ceil(8 Q / 64) store
+// instructions for the wave instead of Q two-byte ones.  The wave's 64 nodes
+// start at a multiple of 64 in one state tile, so every chunk is 16-byte
+// aligned in the destination and in the staging area.
+template <class L, class S>
+__device__ __forceinline__ void store_wave_staged(const KParams& p, const FusedIO<S>& io, int tn, uint32_t l0,
+                                                  const typename S::raw* vals) {
+  using R = typename S::raw;
+  constexpr int Q = L::Q;
+  __shared__ __align__(16) R stage[kBlock / 64][Q * 64];
+  R* st = stage[threadIdx.x >> 6];
+  const int lane = threadIdx.x & 63;
+  static_for<0, Q>([&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    st[K * 64 + lane] = vals[K];
+  });
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int ts = p.ts;
+  char* tile = (char*)io.dst + ((int64_t)Q + (int64_t)tn * Q) * ((int64_t)sizeof(R) << ts);
+#pragma unroll
+  for (int it = 0; it < (8 * Q + 63) / 64; ++it) {
+    const int c = it * 64 + lane;
+    if (c < 8 * Q) {
+      const int K = c >> 3, part = c & 7;
+      typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+      const u32x4 v = *(const u32x4*)(st + K * 64 + part * 8);
+      u32x4* d = (u32x4*)(tile + ((int64_t)K << ts) * (int64_t)sizeof(R) + (int64_t)(l0 + part * 8) * sizeof(R));
+      __builtin_nontemporal_store(v, d);
+    }
+  }
+}
+
+// The fused step of a 16-bit state in deviation arithmetic: the populations
+// stay scaled deviations from the load to the store.  The collision and store
+// are written out here as in collide_store_dev: with the call, the compiler
+// merged the multi-lattice kernel's pull paths differently and that kernel
+// ran 40 % slower on the MI355X.
 template <class L, class S, bool FORCE>
 __device__ __forceinline__ void fused_node_dev(const KParams& p, int64_t node, const FusedIO<S>& io,
                                                bool new_here, bool old_here, uint32_t seed_mix) {
@@ -233,22 +275,32 @@ __device__ __forceinline__ void fused_node_dev(const KParams& p, int64_t node, c
   constexpr int Q = L::Q;
   const int ts = p.ts;
   const uint32_t tmask = (1u << ts) - 1u;
-  const int tn = (__builtin_amdgcn_readfirstlane((int)node) & ~63) >> ts;
+  const int nw = __builtin_amdgcn_readfirstlane((int)node) & ~63;  // the wave's first node
+  const int tn = nw >> ts;
   const uint32_t lb = ((uint32_t)node & tmask) * (uint32_t)sizeof(R);
+  R vals[Q];
   if (p.sr) {
     const uint32_t h = mix32((uint32_t)node ^ seed_mix);
     uint32_t pair = 0;
     static_for<0, Q>([&](auto kc) {
       constexpr int K = decltype(kc)::value;
-      char* base = (char*)io.dst + ((int64_t)Q + (int64_t)tn * Q + K) * ((int64_t)1 << ts) * (int64_t)sizeof(R);
       pair = S::template cvt_sr_half<(K & 1) != 0>(pair, g[K], __umul24(h, dither_mult(K)));
-      __builtin_nontemporal_store((R)(K & 1 ? pair >> 16 : pair), (R*)(base + lb));
+      vals[K] = (R)(K & 1 ? pair >> 16 : pair);
     });
   } else {
     static_for<0, Q>([&](auto kc) {
       constexpr int K = decltype(kc)::value;
+      vals[K] = S::from_float(g[K]);
+    });
+  }
+  if ((int64_t)nw + 64 <= p.n) {
+    store_wave_staged<L, S>(p, io, tn, (uint32_t)nw & tmask, vals);
+  } else {
+    // a partial last wave: per-lane stores of the nodes it holds
+    static_for<0, Q>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
       char* base = (char*)io.dst + ((int64_t)Q + (int64_t)tn * Q + K) * ((int64_t)1 << ts) * (int64_t)sizeof(R);
-      __builtin_nontemporal_store(S::from_float(g[K]), (R*)(base + lb));
+      __builtin_nontemporal_store(vals[K], (R*)(base + lb));
     });
   }
 }

@@ -314,7 +314,7 @@ def run_oracle_rounded(q, size, mu, bcs, steps, f0, storage, body_force=None, sc
 
 
 @pytest.mark.parametrize("rounding", ["nearest", "stochastic"])
-@pytest.mark.parametrize("size", [(9, 10, 15), (11, 9, 13)])
+@pytest.mark.parametrize("size", [(9, 10, 15), (11, 9, 13), (11, 9, 149)])
 @pytest.mark.parametrize("storage", ["f16", "bf16"])
 @pytest.mark.parametrize("q", [19, 27])
 def test_deviation_storage_matches_rounded_oracle(q, storage, size, rounding):
@@ -325,7 +325,9 @@ def test_deviation_storage_matches_rounded_oracle(q, storage, size, rounding):
     storage's 2^-11 or 2^-8), so a rounding of the state can land on the other
     side now and then: after 30 steps f stays within two storage ulps of the
     largest deviation |f - w| of the run, the initial one (relative to
-    max|f|)."""
+    max|f|).  The 12 x 10 x 150 lattice takes 4096-node state tiles, whose
+    inner waves pull through LDS (the smaller ones straddle a tile edge in
+    every wave)."""
     bcs = BC_SETS["channel"]
     nodes = tuple(s + 1 for s in size)
     f0 = random_state(q, nodes, seed=6)
