@@ -82,6 +82,47 @@ __device__ __forceinline__ void step_node(const KParams& p, int64_t node, const 
   }
 }
 
+// Store of a full wave's 16-bit populations through LDS: each lane writes
+// its Q values into the wave's staging area (direction-major, 64 nodes per
+// direction), then the wave stores 16-byte chunks (8 consecutive nodes of one
+// direction) with one global_store_dwordx4 per lane: ceil(8 Q / 64) store
+// instructions for the wave instead of Q two-byte ones (float32 states gain
+// nothing from it on the MI355X).  The wave's 64 nodes start at a multiple
+// of 64 in one state tile, so every chunk is 16-byte aligned in the
+// destination and in the staging area.
+template <class L, class S>
+__device__ __forceinline__ void store_wave_staged(const KParams& p, const FusedIO<S>& io, int tn, uint32_t l0,
+                                                  const typename S::raw* vals) {
+  using R = typename S::raw;
+  constexpr int Q = L::Q;
+  __shared__ __align__(16) R stage[kBlock / 64][Q * 64];
+  R* st = stage[threadIdx.x >> 6];
+  const int lane = threadIdx.x & 63;
+  static_for<0, Q>([&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    st[K * 64 + lane] = vals[K];
+  });
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int ts = p.ts;
+  char* tile = (char*)io.dst + ((int64_t)Q + (int64_t)tn * Q) * ((int64_t)sizeof(R) << ts);
+  constexpr int per = 16 / (int)sizeof(R);  // nodes per chunk
+  constexpr int chunks = 64 / per;            // chunks per direction
+#pragma unroll
+  for (int it = 0; it < (chunks * Q + 63) / 64; ++it) {
+    const int c = it * 64 + lane;
+    if (c < chunks * Q) {
+      const int K = c / chunks, part = c % chunks;
+      typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+      const u32x4 v = *(const u32x4*)(st + K * 64 + part * per);
+      u32x4* d = (u32x4*)(tile + ((int64_t)K << ts) * (int64_t)sizeof(R) + (int64_t)(l0 + part * per) * sizeof(R));
+      __builtin_nontemporal_store(v, d);
+    }
+  }
+}
+
+
 // Collision of a stepped node and the store of its post-collision
 // populations into the destination state (and, next to moving walls, its
 // density for the next step's wall term).
@@ -206,48 +247,6 @@ __device__ __forceinline__ void collide_store_dev(const KParams& p, int64_t node
       char* base = (char*)io.dst + ((int64_t)Q + (int64_t)tn * Q + K) * ((int64_t)1 << ts) * (int64_t)sizeof(R);
       __builtin_nontemporal_store(S::from_float(g[K]), (R*)(base + lb));
     });
-  }
-}
-
-// The fused step of a 16-bit state in deviation arithmetic: the populations
-// stay scaled deviations from the load to the store.  The collision and store
-// are written out here as in collide_store_dev: with the call, the compiler
-// merged the multi-lattice kernel's pull paths differently and that kernel
-// ran 40 % slower on the MI355X.
-// Store of a full wave's 16-bit populations through LDS: each lane writes
-// its Q values into the wave's staging area (direction-major, 64 nodes per
-// direction), then the wave stores 16-byte chunks (8 consecutive nodes of one
-// direction) with one global_store_dwordx4 per lane: ceil(8 Q / 64) store
-// instructions for the wave instead of Q two-byte ones.  The wave's 64 nodes
-// start at a multiple of 64 in one state tile, so every chunk is 16-byte
-// aligned in the destination and in the staging area.
-template <class L, class S>
-__device__ __forceinline__ void store_wave_staged(const KParams& p, const FusedIO<S>& io, int tn, uint32_t l0,
-                                                  const typename S::raw* vals) {
-  using R = typename S::raw;
-  constexpr int Q = L::Q;
-  __shared__ __align__(16) R stage[kBlock / 64][Q * 64];
-  R* st = stage[threadIdx.x >> 6];
-  const int lane = threadIdx.x & 63;
-  static_for<0, Q>([&](auto kc) {
-    constexpr int K = decltype(kc)::value;
-    st[K * 64 + lane] = vals[K];
-  });
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const int ts = p.ts;
-  char* tile = (char*)io.dst + ((int64_t)Q + (int64_t)tn * Q) * ((int64_t)sizeof(R) << ts);
-#pragma unroll
-  for (int it = 0; it < (8 * Q + 63) / 64; ++it) {
-    const int c = it * 64 + lane;
-    if (c < 8 * Q) {
-      const int K = c >> 3, part = c & 7;
-      typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-      const u32x4 v = *(const u32x4*)(st + K * 64 + part * 8);
-      u32x4* d = (u32x4*)(tile + ((int64_t)K << ts) * (int64_t)sizeof(R) + (int64_t)(l0 + part * 8) * sizeof(R));
-      __builtin_nontemporal_store(v, d);
-    }
   }
 }
 
