@@ -417,7 +417,7 @@ def bench_batch(n_cases=256, steps=3000, seed=0, storage="f16", tick=50, baselin
         del lbm
     mean_s = sum(cnt / len(specs) * float(np.mean([per_case[id(specs[i])] for i in picks]))
                  for _, cnt, picks in sample)
-    base_solves = 1.0 / mean_s if sample else float("nan")
+    base_solves = 1.0 / mean_s if sample else None
     n_base = sum(len(p) for _, _, p in sample)
     sizes = [specs[i].n for _, _, p in sample for i in p] or [0]
     solves = n_cases / t
@@ -433,11 +433,11 @@ def bench_batch(n_cases=256, steps=3000, seed=0, storage="f16", tick=50, baselin
         "wall_seconds": round(wall, 3),
         "solves_per_s": round(solves, 2),
         "mlups": round(lups / t / 1e6, 1),
-        "one_at_a_time_solves_per_s": round(base_solves, 3),
+        "one_at_a_time_solves_per_s": round(base_solves, 3) if base_solves else None,
         "one_at_a_time_sample": f"{n_base} cases ({', '.join(f'{len(p)} D3Q{q}' for q, _, p in sample)}; "
                                 f"{min(sizes)} to {max(sizes)} nodes, evenly spaced size quantiles) x "
                                 f"{baseline_steps} steps with LBMD3, scaled to {steps} steps, weighted by lattice",
-        "speedup_vs_one_at_a_time": round(solves / base_solves, 2),
+        "speedup_vs_one_at_a_time": round(solves / base_solves, 2) if base_solves else None,
         "grid": dict(eng.grid),
         "autotune": tune,
     }
@@ -579,6 +579,12 @@ def main():
                     help="skip the D3Q27 suspension precision study")
     ap.add_argument("--precision-steps", type=int, default=2000)
     args = ap.parse_args()
+    t_start = time.perf_counter()
+
+    def done(leg):
+        # one line per finished leg on stderr, so a long run shows progress
+        print(f"[bench] {leg} done at {time.perf_counter() - t_start:.1f} s", file=sys.stderr, flush=True)
+
     if args.gpus != 1:
         print(
             "bench.py measures one GPU: a lattice runs on one GPU and there is no multi-GPU "
@@ -635,6 +641,7 @@ def main():
     }
     del lbm
     torch.cuda.empty_cache()
+    done("headline")
     if not args.no_f16 and args.storage == "f32":
         lbm16 = build_solver(args.size, args.q, "f16")
         # a fresh 16-bit state settles over its first few dozen launches
@@ -654,6 +661,7 @@ def main():
         out["f16_deviation"]["rounding"] = "stochastic"
         out["f16_nearest"] = storage_leg(args.size, args.q, "f16", args.steps, args.warmup, mlups,
                                          rounding="nearest")
+        done("f16")
     if not args.no_extra and args.storage == "f32":
         try:
             out["bf16_deviation"] = storage_leg(args.size, args.q, "bf16", args.steps, args.warmup, mlups)
@@ -669,6 +677,7 @@ def main():
         except Exception as e:  # reported, never silent
             out["extra_error"] = repr(e)
         torch.cuda.empty_cache()
+        done("bf16, D3Q27, suspension and free-fall")
     if not args.no_batch:
         try:
             out["batched"] = bench_batch(n_cases=args.batch_cases, steps=args.batch_steps,
@@ -676,6 +685,7 @@ def main():
         except Exception as e:  # reported, never silent
             out["batched"] = {"error": repr(e)}
         torch.cuda.empty_cache()
+        done("batched")
     if not args.no_serve:
         lo, hi = (int(v) for v in args.serve_steps.split(","))
         rates = [float(v) for v in args.serve_rates.split(",") if v]
@@ -684,6 +694,7 @@ def main():
         except Exception as e:  # reported, never silent
             out["serving"] = {"error": repr(e)}
         torch.cuda.empty_cache()
+        done("serving")
     if not args.no_precision:
         try:
             out["suspension_precision"] = bench_suspension_precision(size=args.size,
@@ -691,6 +702,7 @@ def main():
         except Exception as e:  # reported, never silent
             out["suspension_precision"] = {"error": repr(e)}
         torch.cuda.empty_cache()
+        done("suspension precision")
     if not args.no_copy:
         cb = copy_bandwidth()
         out["roofline"]["copy_peak_gbs"] = round(cb, 1)

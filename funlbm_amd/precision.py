@@ -4,8 +4,8 @@ Each study runs one configuration through LBMD3 once per storage format
 (float64 populations and arithmetic as the reference solution, then float32,
 f16 and bf16 deviation storage with float32 arithmetic, the 16-bit ones with
 round-to-nearest stores and, as "f16-sr" / "bf16-sr", stochastic ones) and
-reports, per format, the relative L-inf error of the velocity field and the
-largest particle-centroid distance from the float64 run.
+reports, per format, the relative L-inf and L2 errors of the velocity field
+and the largest particle-centroid distance from the float64 run.
 """
 
 import math
@@ -41,6 +41,7 @@ def compare(cfg_json: dict, steps: int, storages=("f32", "f16", "bf16", "f16-sr"
     ref["flow"]["storage"] = "f64"
     u64, cx64 = _run(ref, steps)
     scale = float(np.max(np.abs(u64)))
+    norm = float(np.linalg.norm(u64))
     out = {"steps": steps, "max_abs_u_f64": scale}
     for st in storages:
         c = copy.deepcopy(cfg_json)
@@ -50,7 +51,8 @@ def compare(cfg_json: dict, steps: int, storages=("f32", "f16", "bf16", "f16-sr"
         u, cx = _run(c, steps)
         err = float(np.max(np.abs(u - u64)) / scale) if scale > 0 else float(np.max(np.abs(u - u64)))
         cxe = max((float(np.max(np.abs(a - b))) for a, b in zip(cx, cx64)), default=0.0)
-        out[st] = {"u_rel_linf": err, "centroid_abs": cxe}
+        l2 = float(np.linalg.norm(u - u64))
+        out[st] = {"u_rel_linf": err, "u_rel_l2": l2 / norm if norm > 0 else l2, "centroid_abs": cxe}
     return out
 
 
