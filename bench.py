@@ -513,10 +513,64 @@ def serve_error_check(seed=0, steps=(1000, 20000), storage="f16"):
     }
 
 
+def serve_one_at_a_time(seed=0, steps=(1000, 20000), storage="f16", per_lattice=8, timed_steps=200,
+                        rate=50.0, duration=4.0):
+    """Plain one-case-at-a-time solves of the serving mix: requests of the
+    stream at `rate` for `duration` s, `per_lattice` per lattice at evenly
+    spaced size quantiles (smallest and largest included), each solved alone
+    with LBMD3 (eager steps, same storage); a request's time is its measured
+    step time times its step count, and the mix's mean weights each lattice's
+    sample by its share of the stream."""
+    import torch
+
+    from funlbm_amd.batch.workloads import lbm_config_for, serving_requests
+    from funlbm_amd.lbm import LBMD3, Config
+
+    specs = [s for _, s in serving_requests(rate, duration, seed=seed, steps=steps)]
+    mean_s = 0.0
+    picked = []
+    for q in sorted({s.q for s in specs}):
+        ids = sorted((i for i, s in enumerate(specs) if s.q == q), key=lambda i: specs[i].n)
+        k = min(per_lattice, len(ids))
+        picks = sorted({ids[round(j * (len(ids) - 1) / max(k - 1, 1))] for j in range(k)})
+        times = []
+        for i in picks:
+            spec = specs[i]
+            lbm = LBMD3(config=Config().from_json(lbm_config_for(spec, storage)))
+            lbm.init()
+            for n in range(1, 11):
+                lbm.step = n
+                lbm.run_step()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for n in range(11, 11 + timed_steps):
+                lbm.step = n
+                lbm.run_step()
+            torch.cuda.synchronize()
+            times.append((time.perf_counter() - t0) / timed_steps * spec.steps)
+            picked.append(spec)
+            del lbm
+            torch.cuda.empty_cache()
+        mean_s += len(ids) / len(specs) * float(np.mean(times))
+    sizes = [s.nodes[0] for s in picked]
+    return {
+        "solves_per_s": round(1.0 / mean_s, 3),
+        "mean_solve_s": round(mean_s, 3),
+        "sample": f"{len(picked)} requests of the {rate:g} req/s stream ({per_lattice} per lattice, sides "
+                  f"{min(sizes)}-{max(sizes)}, evenly spaced size quantiles), each alone with LBMD3 "
+                  f"for {timed_steps} timed steps, scaled to its step count, weighted by lattice",
+    }
+
+
 def bench_serve(rates, duration, steps=(1000, 20000), budget=200e9, storage="f16", tick=50, drain_s=6.0,
-                max_active_nodes=1 << 25):
+                max_active_nodes=1 << 25, baseline_per_lattice=8):
     sweep = [serve_rate(r, duration, steps=steps, budget=budget, storage=storage, tick=tick, drain_s=drain_s,
                         max_active_nodes=max_active_nodes) for r in rates]
+    base = None
+    if baseline_per_lattice > 0:
+        base = serve_one_at_a_time(steps=steps, storage=storage, per_lattice=baseline_per_lattice)
+        best = max(r["solves_per_s"] for r in sweep)
+        base["speedup_of_best_rate"] = round(best / base["solves_per_s"], 2)
     return {
         "workload": f"Poisson arrivals for {duration} s per rate (run cut {drain_s} s after the last arrival), "
                     f"sides log-uniform 32-128, D3Q19/27, 1-4 spheres or ellipsoids, steps {steps[0]}-{steps[1]}, "
@@ -524,6 +578,7 @@ def bench_serve(rates, duration, steps=(1000, 20000), budget=200e9, storage="f16
                     f"nodes stepping at once (first come, first served), ticks of {tick} steps, "
                     f"two ticks in flight",
         "sweep": sweep,
+        "one_at_a_time": base,
         "error_check": serve_error_check(steps=steps, storage=storage),
     }
 

@@ -85,3 +85,35 @@ def serving_requests(rate, duration, seed=0, side=(32, 128), qs=(19, 27), n_part
         out.append((t, CaseSpec(nodes=nodes, q=q, tau=float(rng.uniform(*tau)), steps=n_steps,
                                 particles=parts, gravity=(0.0, 0.0, gravity))))
     return out
+
+
+FACES = ("input", "output", "back", "forward", "bottom", "top")
+
+
+def lbm_config_for(spec, storage="f16", rounding="stochastic") -> dict:
+    """The LBMD3 configuration (the reference's JSON schema) of one case:
+    the same lattice, walls, relaxation time, gravity and particles, for
+    solving it alone.  A rotation-vector component alpha of 0 is passed as
+    1e-9, since the schema reads a zero alpha as its default pi/2."""
+    from .. import _native as nat
+
+    codes = {nat.BC_WALL: "WALL", nat.BC_PERIODICAL: "PERIODICAL"}
+    parts = []
+    for p in spec.particles:
+        coord = {"center": [float(v) for v in p.center], "alpha": float(p.angle[0]) or 1e-9,
+                 "beta": float(p.angle[1]), "gamma": float(p.angle[2])}
+        d = {"rou": float(p.rou), "coord": coord, "markers": p.markers}
+        if p.kind == "sphere":
+            d.update(type="sphere", r=float(p.r))
+        else:
+            d.update(type="ellipsoid", a=float(p.abc[0]), b=float(p.abc[1]), c=float(p.abc[2]))
+        parts.append(d)
+    return {
+        "flow": {"param_type": f"D3Q{spec.q}", "size": [int(n) - 1 for n in spec.nodes],
+                 "param": {"mu": (float(spec.tau) - 0.5) / 3, "gravity": [float(g) for g in spec.gravity]},
+                 "boundary": {f: {"code": codes[int(spec.bcs[i])]} for i, f in enumerate(FACES)},
+                 "storage": storage, "rounding": rounding, "seed": int(spec.seed), "ibm_delta": spec.delta,
+                 "uzero_faces": [f for i, f in enumerate(FACES) if (spec.uzero_mask >> i) & 1]},
+        "particles": parts,
+        "log_every": 0,
+    }

@@ -465,37 +465,72 @@ __device__ inline T moving_wall_term(const KParams& p, int face, T rho) {
 // The z-face override of a node whose x and y are inner (only the bottom or
 // top rule of source_of can apply there): directions with e_z = DZ take, on a
 // periodic face, plane K at the same (x, y) on the opposite z face; on a wall,
-// plane rev(K) at the node itself, less the moving-wall term.
+// plane rev(K) at the node itself, less the moving-wall term.  The lanes on
+// the face issue the loads of all those directions before the first use, so
+// a wave waits for one round trip, not one per direction; the values then go
+// into f by a select, outside the guarded block (a store into f under the
+// guard lets the compiler merge the two faces' stores through a pointer
+// select, which puts the array in private memory).
+template <class L, int DZ>
+struct ZDirs {
+  static constexpr int count() {
+    int n = 0;
+    for (int k = 0; k < L::Q; ++k) n += L::E[k][2] == DZ;
+    return n;
+  }
+  // the i-th direction with e_z = DZ
+  static constexpr int dir(int i) {
+    for (int k = 0; k < L::Q; ++k)
+      if (L::E[k][2] == DZ && i-- == 0) return k;
+    return -1;
+  }
+};
+
 template <class L, class S, int DZ>
 __device__ inline void zface_pull(const KParams& p, const typename S::elem* __restrict__ src, int64_t node,
                                   const float* __restrict__ rho_prev, bool hit, typename S::T* f) {
   using T = typename S::T;
+  using E = typename S::elem;
+  using Z = ZDirs<L, DZ>;
+  constexpr int NZ = Z::count();
   constexpr int face = DZ > 0 ? 4 : 5;
   const int bc = p.bc[face];
-  static_for<0, L::Q>([&](auto kc) {
-    constexpr int K = decltype(kc)::value;
-    if constexpr (L::E[K][2] == DZ) {
-      // the store is unconditional and only the load is guarded: a guarded
-      // store into the array lets the compiler merge the two faces' stores
-      // through a pointer select, which puts the array in private memory
-      T v = f[K];
-      if (hit) {
-        const float wr = wt<float>(L::W[K]);
-        if (bc == FUNLBM_BC_PERIODICAL) {
-          const int64_t idx = state_index<L::Q>(p, K, node + (DZ > 0 ? (p.nz - 1) : -(p.nz - 1)));
-          v = S::load(src[idx], wr, p.inv_scale, p.scale);
-        } else {
-          v = S::load(src[state_index<L::Q>(p, L::REV[K], node)], wr, p.inv_scale, p.scale);
-          if (bc == FUNLBM_BC_WALL_WITH_SPEED) {
-            if constexpr (S::dev)
-              v = v - moving_wall_term<L, K, T>(p, face, (T)rho_prev[node]) * p.scale;
-            else
-              v = v - moving_wall_term<L, K, T>(p, face, (T)rho_prev[node]);
-          }
-        }
-      }
-      f[K] = v;
+  T rep[NZ];
+  if (hit) {
+    E raw[NZ];
+    if (bc == FUNLBM_BC_PERIODICAL) {
+      const int64_t across = node + (DZ > 0 ? (p.nz - 1) : -(p.nz - 1));
+      static_for<0, NZ>([&](auto ic) {
+        constexpr int I = decltype(ic)::value;
+        raw[I] = src[state_index<L::Q>(p, Z::dir(I), across)];
+      });
+    } else {
+      static_for<0, NZ>([&](auto ic) {
+        constexpr int I = decltype(ic)::value;
+        raw[I] = src[state_index<L::Q>(p, L::REV[Z::dir(I)], node)];
+      });
     }
+    static_for<0, NZ>([&](auto ic) {
+      constexpr int I = decltype(ic)::value;
+      constexpr int K = Z::dir(I);
+      rep[I] = S::load(raw[I], wt<float>(L::W[K]), p.inv_scale, p.scale);
+    });
+    if (bc == FUNLBM_BC_WALL_WITH_SPEED) {
+      const T rho = (T)rho_prev[node];
+      static_for<0, NZ>([&](auto ic) {
+        constexpr int I = decltype(ic)::value;
+        constexpr int K = Z::dir(I);
+        if constexpr (S::dev)
+          rep[I] = rep[I] - moving_wall_term<L, K, T>(p, face, rho) * p.scale;
+        else
+          rep[I] = rep[I] - moving_wall_term<L, K, T>(p, face, rho);
+      });
+    }
+  }
+  static_for<0, NZ>([&](auto ic) {
+    constexpr int I = decltype(ic)::value;
+    constexpr int K = Z::dir(I);
+    f[K] = hit ? rep[I] : f[K];
   });
 }
 

@@ -96,3 +96,35 @@ def test_cpu_device_is_refused():
 
     with pytest.raises(RuntimeError, match="GPU"):
         device_detect("cpu")
+
+
+def test_case_config_parses_to_the_same_case():
+    """lbm_config_for writes a batched case (spheres and ellipsoids, walls,
+    gravity, tau) in the reference's schema; parsed back, the solver's
+    configuration holds the same lattice, faces and particles."""
+    import math
+
+    from funlbm_amd.batch.engine import CaseSpec, ParticleSpec
+    from funlbm_amd.batch.workloads import lbm_config_for
+    from funlbm_amd.lbm import Config
+
+    spec = CaseSpec(nodes=(40, 36, 44), q=27, tau=0.7, steps=10, gravity=(0.0, 0.0, 3e-5),
+                    particles=[ParticleSpec(kind="sphere", r=4.0, rou=1.4, center=(12.0, 11.0, 25.0)),
+                               ParticleSpec(kind="ellipsoid", abc=(3.0, 3.0, 5.0), rou=2.1, center=(27.0, 24.0, 26.5),
+                                            angle=(0.0, 0.4, -1.1), markers="fibonacci")],
+                    uzero_mask=0x3F)
+    cfg = Config().from_json(lbm_config_for(spec, "f16", "nearest"))
+    fc = cfg.flow_config
+    assert fc.param_type == "D3Q27"
+    assert list(fc.size) == [39, 35, 43]
+    assert math.isclose(fc.mu, 0.2 / 3)
+    assert fc.storage == "f16" and fc.rounding == "nearest"
+    assert list(fc.gravity) == [0.0, 0.0, 3e-5]
+    from funlbm_amd.config import BoundaryCondition
+
+    assert all(getattr(fc.boundary, f).is_condition(BoundaryCondition.WALL)
+               for f in ("input", "output", "back", "forward", "bottom", "top"))
+    ps = cfg.particles
+    assert [p.type for p in ps] == ["sphere", "ellipsoid"]
+    assert ps[1].markers == "fibonacci"
+    assert ps[1].coord_config.alpha == 1e-9 and ps[1].coord_config.beta == 0.4 and ps[1].coord_config.gamma == -1.1

@@ -353,3 +353,29 @@ def test_settling_batch_of_256_cases():
     assert ef < 1e-5 and ec < 1e-6, (ef, ec)
     del eng
     torch.cuda.empty_cache()
+
+
+def test_batch_case_with_ellipsoid_equals_single_lattice():
+    """A case with a sphere and a rotated ellipsoid (D3Q27) equals the same
+    case solved alone by LBMD3 from lbm_config_for, bit for bit (float32)."""
+    from funlbm_amd.batch.workloads import lbm_config_for
+    from funlbm_amd.lbm import LBMD3, Config
+
+    spec = CaseSpec(nodes=(34, 30, 38), q=27, tau=0.7, steps=20, gravity=(0.0, 0.0, 2e-5),
+                    particles=[ParticleSpec(kind="sphere", r=3.5, rou=1.6, center=(10.0, 9.5, 22.0)),
+                               ParticleSpec(kind="ellipsoid", abc=(2.5, 2.5, 4.0), rou=2.2,
+                                            center=(23.0, 20.0, 21.0), angle=(0.3, -0.7, 1.2))])
+    lbm = LBMD3(config=Config().from_json(lbm_config_for(spec, "f32", "nearest")))
+    lbm.init()
+    spec.tau = lbm.flow.lat.tau  # the solver's tau comes from mu; use the same float32
+    eng = BatchEngine(storage="f32", max_nodes=1 << 16, max_markers=8192, max_particles=8, max_cases=4)
+    cid = eng.add(spec)
+    eng.step(20)
+    for s in range(1, 21):
+        lbm.step = s
+        lbm.run_step()
+    u, rho = eng.fields(cid)
+    rows = eng.particle_rows(cid)
+    assert torch.equal(rows[:, :15], lbm.particle_swarm.pstate[:, :15])
+    assert torch.equal(u, lbm.flow.u)
+    assert torch.equal(rho, lbm.flow.rou)
