@@ -608,21 +608,33 @@ __device__ inline void pull_node(const KParams& p, const typename S::elem* __res
     zface_pull<L, S, -1>(p, src, node, rho_prev, c.z == p.nz - 1, f);
     return;
   }
+  // every direction's load is issued before the first is used (one round
+  // trip per node, not one per direction); the moving-wall terms follow
+  E raw[Q];
   static_for<0, Q>([&](auto kc) {
     constexpr int K = decltype(kc)::value;
     const Src s = source_of<L, K>(p, c);
-    const int64_t idx = state_index<Q>(p, s.k, lin(p, s.x, s.y, s.z));
-    // The element was stored with the weight of s.k; rev(k) and k share a
-    // weight on every lattice.
-    T v = S::load(src[idx], wt<float>(L::W[K]), p.inv_scale, p.scale);
-    if (s.moving_face >= 0) {
-      if constexpr (S::dev)
-        v = v - moving_wall_term<L, K, T>(p, s.moving_face, (T)rho_prev[node]) * p.scale;
-      else
-        v = v - moving_wall_term<L, K, T>(p, s.moving_face, (T)rho_prev[node]);
-    }
-    f[K] = v;
+    raw[K] = src[state_index<Q>(p, s.k, lin(p, s.x, s.y, s.z))];
   });
+  // The element was stored with the weight of s.k; rev(k) and k share a
+  // weight on every lattice.
+  static_for<0, Q>([&](auto kc) {
+    constexpr int K = decltype(kc)::value;
+    f[K] = S::load(raw[K], wt<float>(L::W[K]), p.inv_scale, p.scale);
+  });
+  if (p.has_moving) {
+    const T rho = (T)rho_prev[node];
+    static_for<0, Q>([&](auto kc) {
+      constexpr int K = decltype(kc)::value;
+      const Src s = source_of<L, K>(p, c);
+      if (s.moving_face >= 0) {
+        if constexpr (S::dev)
+          f[K] = f[K] - moving_wall_term<L, K, T>(p, s.moving_face, rho) * p.scale;
+        else
+          f[K] = f[K] - moving_wall_term<L, K, T>(p, s.moving_face, rho);
+      }
+    });
+  }
 }
 
 // ---------------------------------------------------------------------------
