@@ -217,6 +217,12 @@ class BatchEngine:
         self.traj_cap_t = torch.tensor([self.traj_cap], device=dev, dtype=i32)
         self.slots = {}
         self.grid = {"tiles": 2048, "markers": 1024, "particles": 256}
+        # exact_grids: every eager launch takes its grids from the live counts
+        # (one block per active tile of each lattice; the marker and particle
+        # grids as autotune sets them).  A captured graph keeps the grids of
+        # its capture, so this suits eager stepping of a changing population.
+        self.exact_grids = False
+        self._cus = torch.cuda.get_device_properties(dev).multi_processor_count
         self._graph = None
         self._graph_steps = 0
         self._graph_mask = 0
@@ -400,14 +406,27 @@ class BatchEngine:
     def _lattice_grid(self):
         """Grid of each lattice's fused kernel: its own tile count, at least
         MIN_LATTICE_GRID blocks (cases admitted after a graph capture are
-        then still spread over the chip), at most grid["tiles"]."""
+        then still spread over the chip), at most grid["tiles"]; with
+        exact_grids, its own tile count."""
+        if self.exact_grids:
+            return (ctypes.c_int32 * 4)(*[max(1, int(c)) for c in self._lattice_tiles])
         cap = int(self.grid["tiles"])
         return (ctypes.c_int32 * 4)(*[max(1, min(cap, max(int(c), MIN_LATTICE_GRID)))
                                       for c in self._lattice_tiles])
 
+    def _ibm_grids(self):
+        """Marker and particle grids for the current counts: about 16 markers
+        per block and one block per particle, each at most a few blocks per
+        compute unit (the kernels are grid-stride loops)."""
+        n_mark = max(1, self.markers.high_water)
+        n_part = max(1, self.parts.high_water)
+        return max(1, min(-(-n_mark // 16), 8 * self._cus)), max(1, min(n_part, 4 * self._cus))
+
     def _launch(self):
         P = nat.ptr
-        g = self.grid
+        g = dict(self.grid)
+        if self.exact_grids:
+            g["markers"], g["particles"] = self._ibm_grids()
         nat.call(
             "funlbm_batch_step", self.storage, P(self.cases), P(self.hdr), P(self.case_par),
             P(self.case_left), P(self.case_done), P(self.case_flag), P(self.tile_case),
@@ -435,11 +454,9 @@ class BatchEngine:
         timings; the winner stays in self.grid."""
         self._publish()
         dev = self.device
-        cus = torch.cuda.get_device_properties(dev).multi_processor_count
+        cus = self._cus
         n_tiles = int(self.hdr[H_TILES].item())
-        n_mark = max(1, int(self.hdr[H_MARKERS].item()))
-        self.grid["markers"] = max(1, min(-(-n_mark // 16), 8 * cus))
-        self.grid["particles"] = max(1, min(int(self.hdr[H_PARTICLES].item()), 4 * cus))
+        self.grid["markers"], self.grid["particles"] = self._ibm_grids()
         if candidates is None:
             candidates = sorted({c for c in (2 * cus, 4 * cus, 8 * cus, 16 * cus, n_tiles) if 0 < c <= max(n_tiles, 1)})
         timings = {}

@@ -4,9 +4,13 @@ Requests arrive over time (each a CaseSpec: lattice, particles, step count).
 The server keeps their state resident in the BatchEngine's pools and admits a
 waiting request as soon as its HBM footprint fits under the budget and the
 pools have room; admitted cases step together, each to its own step count, in
-ticks of `tick_steps` steps (one replay of a captured HIP graph whose kernels
-read the live work lists from device memory, so admissions between ticks need
-no re-capture).
+ticks of `tick_steps` steps.  By default a tick is `tick_steps` eager step
+launches whose grids the engine sizes from the live counts (one block per
+active tile of each lattice: the population changes between ticks, and a
+persistent grid of fixed size ran the fused kernels 15 % slower).  With
+`graph=True` a tick is one replay of a captured HIP graph whose kernels read
+the live work lists from device memory, so admissions between ticks need no
+re-capture, but the grids stay those of the capture.
 
 The loop is pipelined: up to `depth` ticks are in flight on the stream.
 Behind each tick the server enqueues asynchronous copies of the per-case
@@ -79,12 +83,15 @@ class _Snapshot:
 
 class Server:
     def __init__(self, engine: BatchEngine, hbm_budget: float, tick_steps: int = 50,
-                 stream_trajectories: bool = True, depth: int = 2, max_active_nodes: Optional[int] = None):
+                 stream_trajectories: bool = True, depth: int = 2, max_active_nodes: Optional[int] = None,
+                 graph: bool = False):
         if engine.traj_cap < tick_steps:
             raise ValueError("the trajectory ring must hold one tick of steps")
         if depth < 1:
             raise ValueError("at least one tick must be in flight")
         self.engine = engine
+        self.graph = bool(graph)
+        engine.exact_grids = not self.graph
         self.budget = float(hbm_budget)
         self.tick_steps = int(tick_steps)
         self.stream = stream_trajectories
@@ -131,10 +138,13 @@ class Server:
     def _launch(self):
         """Enqueues one tick and the snapshot copies behind it."""
         eng = self.engine
-        if not self._graph_ready:
-            eng.capture(self.tick_steps)
-            self._graph_ready = True
-        eng.run_graph()
+        if not self.graph:
+            eng.step(self.tick_steps)
+        else:
+            if not self._graph_ready:
+                eng.capture(self.tick_steps)
+                self._graph_ready = True
+            eng.run_graph()
         snap = self._free.pop()
         snap.record(eng, self._ticks)
         self._inflight.append(snap)

@@ -36,12 +36,15 @@ def engine():
                        traj_cap=32)
 
 
-def test_server_mix_equals_alone_and_respects_budget():
+@pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
+def test_server_mix_equals_alone_and_respects_budget(graph):
+    """Eager ticks (grids from the live counts) and graph-replayed ticks
+    (grids of the capture) give the same bits as each request alone."""
     reqs = small_requests()
     eng = engine()
     # room for about two of these cases at a time
     budget = 2.2 * max(BatchEngine.case_bytes_needed(r.spec) for r in reqs)
-    srv = Server(eng, hbm_budget=budget, tick_steps=25)
+    srv = Server(eng, hbm_budget=budget, tick_steps=25, graph=graph)
     peak = []
     orig_admit = srv._admit
 
