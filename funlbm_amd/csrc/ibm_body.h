@@ -93,6 +93,7 @@ struct LaneW {
   float wsum;      // sum of the normalised weights of the stencil
 };
 
+// Call with the marker's 16 lanes together (it shuffles and sums within them).
 __device__ inline LaneW lane_weight(const KParams& p, const float x[3], int sub) {
   LaneW lw;
   const Stencil s = stencil_of(p, x);
@@ -104,8 +105,17 @@ __device__ inline LaneW lane_weight(const KParams& p, const float x[3], int sub)
   // unnormalised weight per node: (t_x t_y) t_z, t_y and t_z shared by the layers
   const float ry = (float)lw.y - x[1], rz = (float)lw.z - x[2];
   const bool inyz = fabsf(ry) <= 2.0f && fabsf(rz) <= 2.0f;
-  const float ty = delta_1d(p, ry);
-  const float tz = delta_1d(p, rz);
+  // The marker needs 12 kernel values: t_x of its four layers, t_y of its
+  // four y offsets and t_z of its four z offsets.  Lane 4a + o evaluates
+  // axis a's value at offset o (lanes 12-15 a spare z one), from the same
+  // operands as the lane that uses it, and each lane reads its six values
+  // from those lanes: one kernel evaluation per lane instead of six.
+  const int ax = sub >> 2, o = sub & 3;
+  const float xa = ax == 0 ? x[0] : ax == 1 ? x[1] : x[2];
+  const int la = ax == 0 ? s.lo[0] : ax == 1 ? s.lo[1] : s.lo[2];
+  const float ta = delta_1d(p, (float)(la + o) - xa);
+  const float ty = __shfl(ta, 4 + j, kMarkerLanes);
+  const float tz = __shfl(ta, 8 + k, kMarkerLanes);
   lw.valid = 0;
   float w[4];
 #pragma unroll
@@ -113,7 +123,7 @@ __device__ inline LaneW lane_weight(const KParams& p, const float x[3], int sub)
     const bool v = vyz && lw.x0 + i < s.hi[0];
     lw.valid |= (unsigned)v << i;
     const float rx = (float)(lw.x0 + i) - x[0];
-    const float tx = delta_1d(p, rx);
+    const float tx = __shfl(ta, i, kMarkerLanes);
     w[i] = (v && inyz && fabsf(rx) <= 2.0f) ? (tx * ty) * tz : 0.0f;
   }
   const float ws = stencil_sum(w);
