@@ -53,3 +53,24 @@ def test_settling_sphere_centroid_errors():
     for st in ("bf16", "bf16-sr"):
         assert res[st]["u_rel_linf"] < 5e-3, st
         assert res[st]["centroid_abs"] < 1e-6, st
+
+
+def test_dense_suspension_storage_errors():
+    """The dense-suspension study at reduced size (D3Q27 48^3 periodic, three
+    (4, 4, 8) ellipsoids, 500 steps).  Here the 16-bit states miss 1e-3: the
+    reference zeroes u on every face after each moment update, which on
+    periodic faces launches an acoustic field whose density fluctuation is
+    10-20 times the decaying velocity, and a 16-bit deviation resolves u only
+    to about 2^-11 of that density term.  Float32 populations do worse still
+    (their ulp near w is above these velocities).  The bounds guard the
+    measured errors with a margin; the 1e-3 bar is not met at this size."""
+    cfg = precision.suspension(n=48, n_particles=3, mu=0.05 / 3)
+    res = precision.compare(cfg, steps=500)
+    print(json.dumps(res))
+    assert res["max_abs_u_f64"] > 1e-6
+    for st in ("f16", "f16-sr"):
+        assert res[st]["u_rel_linf"] < 0.05 * res["f32"]["u_rel_linf"], st
+        assert res[st]["u_rel_linf"] < 5e-3, st
+        assert res[st]["centroid_abs"] < 1e-5, st
+    for st in ("bf16", "bf16-sr"):
+        assert res[st]["u_rel_linf"] < 4e-2, st
