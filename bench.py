@@ -568,7 +568,7 @@ def serve_one_at_a_time(seed=0, steps=(1000, 20000), storage="f16", per_lattice=
 
 
 def bench_serve(rates, duration, steps=(1000, 20000), budget=200e9, storage="f16", tick=50, drain_s=6.0,
-                max_active_nodes=1 << 26, baseline_per_lattice=8):
+                max_active_nodes=1 << 26, baseline_per_lattice=8, error_check=True):
     sweep = [serve_rate(r, duration, steps=steps, budget=budget, storage=storage, tick=tick, drain_s=drain_s,
                         max_active_nodes=max_active_nodes) for r in rates]
     base = None
@@ -588,7 +588,7 @@ def bench_serve(rates, duration, steps=(1000, 20000), budget=200e9, storage="f16
                     f"two ticks in flight",
         "sweep": sweep,
         "one_at_a_time": base,
-        "error_check": serve_error_check(steps=steps, storage=storage),
+        "error_check": serve_error_check(steps=steps, storage=storage) if error_check else None,
     }
 
 
@@ -634,6 +634,10 @@ def main():
                     help="seconds of arrivals per rate")
     ap.add_argument("--serve-steps", default="1000,20000", help="min,max steps of a request")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-baseline-per-lattice", type=int, default=8,
+                    help="one-at-a-time sample requests per lattice of the serving leg (0: no baseline)")
+    ap.add_argument("--no-serve-error-check", action="store_true",
+                    help="skip the served storage's float64 comparison")
     ap.add_argument("--no-f16", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the bf16, D3Q27 and suspension legs")
@@ -754,7 +758,9 @@ def main():
         lo, hi = (int(v) for v in args.serve_steps.split(","))
         rates = [float(v) for v in args.serve_rates.split(",") if v]
         try:
-            out["serving"] = bench_serve(rates, args.serve_seconds, steps=(lo, hi))
+            out["serving"] = bench_serve(rates, args.serve_seconds, steps=(lo, hi),
+                                         baseline_per_lattice=args.serve_baseline_per_lattice,
+                                         error_check=not args.no_serve_error_check)
         except Exception as e:  # reported, never silent
             out["serving"] = {"error": repr(e)}
         torch.cuda.empty_cache()
