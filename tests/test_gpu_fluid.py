@@ -287,6 +287,44 @@ def test_fused_bitwise_equal_to_oracle(q, size):
     assert np.array_equal(got.view(np.uint32), st.f.view(np.uint32)), rel_linf(got, st.f)
 
 
+@pytest.mark.parametrize("nodes", [(1, 4, 5), (4, 1, 5), (4, 5, 1), (2, 2, 2), (1, 1, 1), (3, 3, 2), (5, 3, 1)])
+@pytest.mark.parametrize("bcname", ["periodic", "wall", "mixed"])
+@pytest.mark.parametrize("q", [13, 15, 19, 27])
+def test_fused_degenerate_lattices_bitwise(q, bcname, nodes):
+    """One or two nodes along an axis, down to a single node: both faces of
+    such an axis act on the same node (each for the directions entering
+    through it), rows of one node wrap onto themselves, and fewer nodes than
+    a wavefront hold the state.  The float32 fused step stays bit-identical
+    to the oracle's np.roll and face overrides."""
+    bcs = BC_SETS[bcname]
+    size = tuple(n - 1 for n in nodes)
+    f0 = random_state(q, nodes, seed=9)
+    g = [1e-5, -2e-5, 3e-5]
+    lbm = run_solver(q, size, 0.07, bcs, 12, f0=f0, body_force=g)
+    st = run_oracle(q, size, 0.07, bcs, 12, f0=f0, body_force=g)
+    got = lbm.flow.f.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), st.f.view(np.uint32)), rel_linf(got, st.f)
+
+
+@pytest.mark.parametrize("nodes", [(4, 5, 1), (2, 2, 2), (1, 1, 1)])
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_deviation_storage_degenerate_lattices(storage, nodes):
+    """The 16-bit fused step on lattices smaller than a wavefront (the
+    partial-wave stores) against the oracle with the storage's round trip,
+    within two storage ulps as in the test below."""
+    q, bcs = 19, BC_SETS["mixed"]
+    size = tuple(n - 1 for n in nodes)
+    f0 = random_state(q, nodes, seed=10)
+    g = [2e-5, 0.0, -1e-5]
+    lbm = run_solver(q, size, 0.07, bcs, 20, f0=f0, storage=storage, body_force=g, rounding="nearest")
+    st = run_oracle_rounded(q, size, 0.07, bcs, 20, f0, storage, body_force=g)
+    got = lbm.flow.f.cpu().numpy()
+    bits = {"f16": 11, "bf16": 8}[storage]
+    dev = float(np.max(np.abs(f0 - st.lat.w.astype(np.float32))))
+    tol = 2.0 * 2.0 ** -bits * dev / float(np.max(np.abs(st.f)))
+    assert rel_linf(got, st.f) < tol, (rel_linf(got, st.f), tol)
+
+
 def run_oracle_rounded(q, size, mu, bcs, steps, f0, storage, body_force=None, scale=1024.0,
                        rounding="nearest", seed=0):
     """The oracle's steps with the state storage's rounding applied to the
