@@ -208,3 +208,30 @@ def test_noalloc_step_rejects_lattices_beyond_its_scratch():
                                                    None, None, None)
     assert rc == nat.ETOOBIG
     assert "maximum" in nat.last_error()
+
+
+@pytest.mark.parametrize("q,storage,dims,ok", [
+    (19, 0, (1291, 1291, 1291), False),  # 2 151 685 171 nodes: past 2^31 - 1
+    (19, 0, (1290, 1290, 1290), True),   # 2 146 689 000 nodes: below it
+    (27, 3, (3, 3300, 3300), False),     # float64 D3Q27: a pull offset of 11e6 nodes x 216 B exceeds 2^31
+    (27, 0, (3, 3300, 3300), True),      # the same grid in float32: 11e6 x 108 B fits
+])
+def test_size_limits_are_refused_on_the_host(q, storage, dims, ok):
+    """The node count must stay below 2^31 and every pull source offset
+    within 32 bits; both are checked on the host (funlbm_state_bytes needs no
+    device) with FUNLBM_ETOOBIG."""
+    lat = nat.Lattice()
+    lat.q = q
+    lat.nx, lat.ny, lat.nz = dims
+    lat.tau = 0.8
+    lat.storage = storage
+    lat.dev_scale = 1024.0
+    lat.rho0 = 1.0
+    for i in range(6):
+        lat.bc[i] = nat.BC_PERIODICAL
+    out = ctypes.c_int64(0)
+    rc = nat.lib().funlbm_state_bytes(ctypes.byref(lat), ctypes.byref(out))
+    if ok:
+        assert rc == 0 and out.value > 0
+    else:
+        assert rc == nat.ETOOBIG, nat.last_error()
