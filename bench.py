@@ -444,10 +444,12 @@ def bench_batch(n_cases=256, steps=3000, seed=0, storage="f16", tick=50, baselin
 
 
 def serve_rate(rate, duration, seed=0, steps=(1000, 20000), budget=200e9, storage="f16", tick=50,
-               drain_s=6.0, pool_nodes_cap=1 << 28, max_active_nodes=None):
+               drain_s=6.0, pool_nodes_cap=1 << 28, max_active_nodes=None, engines=3):
     """Poisson stream of settling requests at `rate` req/s for `duration` s,
-    served with continuous batching; the run is cut `drain_s` after the last
-    arrival (requests still queued or running then are the backlog)."""
+    served with continuous batching by `engines` engines on streams of their
+    own (each with pools for the whole stream); the run is cut `drain_s`
+    after the last arrival (requests still queued or running then are the
+    backlog)."""
     import torch
 
     from funlbm_amd.batch.engine import BatchEngine, markers_for, prepare_markers
@@ -459,9 +461,10 @@ def serve_rate(rate, duration, seed=0, steps=(1000, 20000), budget=200e9, storag
     need_nodes = sum(s.n for _, s in reqs)
     pool_nodes = min(need_nodes + 4096 * len(reqs), pool_nodes_cap)
     markers = sum(len(markers_for(p)[0]) for _, s in reqs for p in s.particles)
-    eng = BatchEngine(storage=storage, max_nodes=pool_nodes, max_markers=markers + 64,
-                      max_particles=4 * len(reqs) + 4, max_cases=len(reqs) + 1, max_q=27, traj_cap=tick)
-    srv = Server(eng, hbm_budget=budget, tick_steps=tick, max_active_nodes=max_active_nodes)
+    engs = [BatchEngine(storage=storage, max_nodes=pool_nodes, max_markers=markers + 64,
+                        max_particles=4 * len(reqs) + 4, max_cases=len(reqs) + 1, max_q=27, traj_cap=tick)
+            for _ in range(engines)]
+    srv = Server(engs, hbm_budget=budget, tick_steps=tick, max_active_nodes=max_active_nodes)
     stats = srv.run([Request(rid=i, spec=s, arrival=t) for i, (t, s) in enumerate(reqs)],
                     max_seconds=duration + drain_s)
     # lattice updates done by the GPU in the run, finished or not
@@ -472,7 +475,7 @@ def serve_rate(rate, duration, seed=0, steps=(1000, 20000), budget=200e9, storag
     # window from a cold start over-count short requests and miss the ramp)
     mean_work = float(np.mean([s.n * s.steps for _, s in reqs]))
     stats["solves_per_s_steady"] = stats["glups_stepped"] * 1e9 / mean_work
-    del srv, eng
+    del srv, engs
     torch.cuda.empty_cache()
     out = {"rate": rate}
     for k in ("submitted", "completed", "backlog", "solves_per_s", "solves_per_s_steady", "mlups", "glups_stepped",
@@ -568,9 +571,9 @@ def serve_one_at_a_time(seed=0, steps=(1000, 20000), storage="f16", per_lattice=
 
 
 def bench_serve(rates, duration, steps=(1000, 20000), budget=200e9, storage="f16", tick=50, drain_s=6.0,
-                max_active_nodes=1 << 26, baseline_per_lattice=8, error_check=True):
+                max_active_nodes=1 << 26, baseline_per_lattice=8, error_check=True, engines=3):
     sweep = [serve_rate(r, duration, steps=steps, budget=budget, storage=storage, tick=tick, drain_s=drain_s,
-                        max_active_nodes=max_active_nodes) for r in rates]
+                        max_active_nodes=max_active_nodes, engines=engines) for r in rates]
     base = None
     if baseline_per_lattice > 0:
         base = serve_one_at_a_time(steps=steps, storage=storage, per_lattice=baseline_per_lattice)
@@ -585,7 +588,7 @@ def bench_serve(rates, duration, steps=(1000, 20000), budget=200e9, storage="f16
                     f"sides log-uniform 32-128, D3Q19/27, 1-4 spheres or ellipsoids, steps {steps[0]}-{steps[1]}, "
                     f"{storage}-deviation storage, {budget / 1e9:.0f} GB budget, at most {max_active_nodes} "
                     f"nodes stepping at once (first come, first served), ticks of {tick} steps, "
-                    f"two ticks in flight",
+                    f"two ticks in flight, {engines} engines on streams of their own",
         "sweep": sweep,
         "one_at_a_time": base,
         "error_check": serve_error_check(steps=steps, storage=storage) if error_check else None,
@@ -638,6 +641,8 @@ def main():
                     help="one-at-a-time sample requests per lattice of the serving leg (0: no baseline)")
     ap.add_argument("--no-serve-error-check", action="store_true",
                     help="skip the served storage's float64 comparison")
+    ap.add_argument("--serve-engines", type=int, default=3,
+                    help="batch engines of the server, each on a stream of its own")
     ap.add_argument("--no-f16", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the bf16, D3Q27 and suspension legs")
@@ -760,7 +765,8 @@ def main():
         try:
             out["serving"] = bench_serve(rates, args.serve_seconds, steps=(lo, hi),
                                          baseline_per_lattice=args.serve_baseline_per_lattice,
-                                         error_check=not args.no_serve_error_check)
+                                         error_check=not args.no_serve_error_check,
+                                         engines=args.serve_engines)
         except Exception as e:  # reported, never silent
             out["serving"] = {"error": repr(e)}
         torch.cuda.empty_cache()

@@ -36,15 +36,17 @@ def engine():
                        traj_cap=32)
 
 
-@pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
-def test_server_mix_equals_alone_and_respects_budget(graph):
+@pytest.mark.parametrize("graph,engines", [(False, 1), (True, 1), (False, 2), (True, 2)],
+                         ids=["eager", "graph", "eager-2-engines", "graph-2-engines"])
+def test_server_mix_equals_alone_and_respects_budget(graph, engines):
     """Eager ticks (grids from the live counts) and graph-replayed ticks
-    (grids of the capture) give the same bits as each request alone."""
+    (grids of the capture), with one engine or two on streams of their own,
+    give the same bits as each request alone."""
     reqs = small_requests()
-    eng = engine()
+    engs = [engine() for _ in range(engines)]
     # room for about two of these cases at a time
     budget = 2.2 * max(BatchEngine.case_bytes_needed(r.spec) for r in reqs)
-    srv = Server(eng, hbm_budget=budget, tick_steps=25, graph=graph)
+    srv = Server(engs if engines > 1 else engs[0], hbm_budget=budget, tick_steps=25, graph=graph)
     peak = []
     orig_admit = srv._admit
 
@@ -59,6 +61,7 @@ def test_server_mix_equals_alone_and_respects_budget(graph):
     # requests overlapped: someone was admitted while another was running
     by_adm = sorted(reqs, key=lambda r: r.admitted)
     assert any(b.admitted < a.finished for a, b in zip(by_adm, by_adm[1:]))
+    assert {r.engine for r in reqs} == set(range(engines))
     for r in reqs:
         alone = engine()
         cid = alone.add(r.spec)
@@ -96,8 +99,7 @@ def test_serving_stream_mix_equals_alone_and_f16_error():
     assert {s.q for s in specs} == {19, 27}
     assert {p.kind for s in specs for p in s.particles} == {"sphere", "ellipsoid"}
     reqs = [Request(rid=i, spec=s, arrival=t) for i, (t, s) in enumerate(stream)]
-    eng = _engine_for(specs)
-    srv = Server(eng, hbm_budget=200e9, tick_steps=25)
+    srv = Server([_engine_for(specs), _engine_for(specs)], hbm_budget=200e9, tick_steps=25)
     stats = srv.run(reqs)
     assert stats["completed"] == len(reqs) and stats["backlog"] == 0
     for r in reqs:
