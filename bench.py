@@ -340,6 +340,54 @@ def cpu_baseline(n=256, q=19, budget_s=12.0):
     }
 
 
+def batch_on_streams(specs, n_engines=2, storage="f16", tick=50):
+    """The same cases split over `n_engines` engines (alternately by size),
+    each replaying its own captured graph on a stream of its own, so one
+    engine's latency-bound IBM kernels run beside another's fused kernels.
+    Seconds until every case has taken its steps."""
+    import torch
+
+    from funlbm_amd.batch.engine import BatchEngine, markers_for
+
+    order = sorted(range(len(specs)), key=lambda i: specs[i].n)
+    engs = []
+    for j in range(n_engines):
+        group = [specs[i] for i in order[j::n_engines]]
+        nodes = sum(s.n for s in group)
+        markers = sum(len(markers_for(p)[0]) for s in group for p in s.particles)
+        e = BatchEngine(storage=storage, max_nodes=nodes + 256 * len(group) + 4096, max_markers=markers + 64,
+                        max_particles=len(group) + 1, max_cases=len(group), max_q=19, traj_cap=tick)
+        for s in group:
+            e.add(s)
+        e.exact_grids = True  # one block per tile: the grid the autotuner picks for this workload
+        engs.append(e)
+    streams = [torch.cuda.Stream() for _ in engs]
+    main = torch.cuda.current_stream()
+    for e, st in zip(engs, streams):
+        st.wait_stream(main)
+        with torch.cuda.stream(st):
+            e.capture(tick)
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for st in streams:
+        st.wait_stream(main)
+    rounds = -(-max(s.steps for s in specs) // tick)
+    for _ in range(rounds):
+        for e, st in zip(engs, streams):
+            with torch.cuda.stream(st):
+                e.run_graph()
+    for st in streams:
+        main.wait_stream(st)
+    b.record()
+    torch.cuda.synchronize()
+    assert all(int(e.case_left.max().item()) == 0 for e in engs)
+    del engs
+    torch.cuda.empty_cache()
+    return a.elapsed_time(b) * 1e-3
+
+
 def bench_batch(n_cases=256, steps=3000, seed=0, storage="f16", tick=50, baseline_per_lattice=8,
                 baseline_steps=200):
     """Batched independent settling cases (all cases advance in one persistent
@@ -378,6 +426,7 @@ def bench_batch(n_cases=256, steps=3000, seed=0, storage="f16", tick=50, baselin
     t = a.elapsed_time(b) * 1e-3
     wall = time.perf_counter() - t0
     assert all(eng.left()[c] == 0 for c in eng.slots)
+    grid = dict(eng.grid)
     lups = sum(s.n * s.steps for s in specs)
     # one case at a time: the single-lattice solver, eager, same storage, on
     # a sample of each lattice's cases at evenly spaced size quantiles
@@ -421,6 +470,9 @@ def bench_batch(n_cases=256, steps=3000, seed=0, storage="f16", tick=50, baselin
     n_base = sum(len(p) for _, _, p in sample)
     sizes = [specs[i].n for _, _, p in sample for i in p] or [0]
     solves = n_cases / t
+    del eng
+    torch.cuda.empty_cache()
+    t2 = batch_on_streams(specs, 2, storage=storage, tick=tick)
     return {
         "workload": f"{n_cases} settling cases, sides log-uniform 32-96, D3Q15/19, one sphere each, "
                     f"{steps} steps, {storage}-deviation storage",
@@ -438,8 +490,15 @@ def bench_batch(n_cases=256, steps=3000, seed=0, storage="f16", tick=50, baselin
                                 f"{min(sizes)} to {max(sizes)} nodes, evenly spaced size quantiles) x "
                                 f"{baseline_steps} steps with LBMD3, scaled to {steps} steps, weighted by lattice",
         "speedup_vs_one_at_a_time": round(solves / base_solves, 2) if base_solves else None,
-        "grid": dict(eng.grid),
+        "grid": grid,
         "autotune": tune,
+        "two_streams": {
+            "workload": "the same cases split over two engines, each replaying its own graph on a stream of its "
+                        "own (one engine's IBM kernels beside the other's fused kernels)",
+            "seconds": round(t2, 3),
+            "solves_per_s": round(n_cases / t2, 2),
+            "speedup_vs_one_at_a_time": round(n_cases / t2 / base_solves, 2) if base_solves else None,
+        },
     }
 
 
