@@ -544,15 +544,15 @@ def serve_rate(rate, duration, seed=0, steps=(1000, 20000), budget=200e9, storag
     return out
 
 
-def serve_error_check(seed=0, steps=(1000, 20000), storage="f16"):
-    """The served storage's error against float64 on request 0 of the
+def serve_error_check(seed=0, steps=(1000, 20000), storage="f16", index=0):
+    """The served storage's error against float64 on request `index` of the
     stream, solved alone (float32 storage beside it)."""
     import torch
 
     from funlbm_amd.batch.engine import BatchEngine, markers_for
     from funlbm_amd.batch.workloads import serving_requests
 
-    spec = serving_requests(20.0, 1.0, seed=seed, steps=steps)[0][1]
+    spec = serving_requests(20.0, 1.0, seed=seed, steps=steps)[index][1]
     fields = {}
     for st in ("f64", "f32", storage):
         e = BatchEngine(storage=st, max_nodes=spec.n + 16384,
@@ -569,7 +569,8 @@ def serve_error_check(seed=0, steps=(1000, 20000), storage="f16"):
     u32, c32 = fields["f32"]
     scale = float(np.abs(u64).max())
     return {
-        "sample": f"request 0 alone: {spec.nodes[0]}^3 D3Q{spec.q}, {len(spec.particles)} particles, {spec.steps} steps",
+        "sample": f"request {index} alone: {spec.nodes[0]}^3 D3Q{spec.q}, {len(spec.particles)} particles "
+                  f"({', '.join(p.kind for p in spec.particles)}), {spec.steps} steps",
         "max_abs_u_f64": scale,
         "u_rel_linf_vs_f64": float(np.abs(us - u64).max() / scale) if scale > 0 else None,
         "centroid_abs_vs_f64": float(np.abs(cs - c64).max()) if len(c64) else None,
@@ -651,6 +652,8 @@ def bench_serve(rates, duration, steps=(1000, 20000), budget=200e9, storage="f16
         "sweep": sweep,
         "one_at_a_time": base,
         "error_check": serve_error_check(steps=steps, storage=storage) if error_check else None,
+        # a D3Q27 request of the same stream with four particles
+        "error_check_d3q27": serve_error_check(steps=steps, storage=storage, index=5) if error_check else None,
     }
 
 
