@@ -503,7 +503,7 @@ def bench_batch(n_cases=256, steps=3000, seed=0, storage="f16", tick=50, baselin
 
 
 def serve_rate(rate, duration, seed=0, steps=(1000, 20000), budget=200e9, storage="f16", tick=50,
-               drain_s=6.0, pool_nodes_cap=1 << 28, max_active_nodes=None, engines=3):
+               drain_s=6.0, pool_nodes_cap=1 << 28, max_active_nodes=None, engines=3, depth=2):
     """Poisson stream of settling requests at `rate` req/s for `duration` s,
     served with continuous batching by `engines` engines on streams of their
     own (each with pools for the whole stream); the run is cut `drain_s`
@@ -523,7 +523,7 @@ def serve_rate(rate, duration, seed=0, steps=(1000, 20000), budget=200e9, storag
     engs = [BatchEngine(storage=storage, max_nodes=pool_nodes, max_markers=markers + 64,
                         max_particles=4 * len(reqs) + 4, max_cases=len(reqs) + 1, max_q=27, traj_cap=tick)
             for _ in range(engines)]
-    srv = Server(engs, hbm_budget=budget, tick_steps=tick, max_active_nodes=max_active_nodes)
+    srv = Server(engs, hbm_budget=budget, tick_steps=tick, max_active_nodes=max_active_nodes, depth=depth)
     stats = srv.run([Request(rid=i, spec=s, arrival=t) for i, (t, s) in enumerate(reqs)],
                     max_seconds=duration + drain_s)
     # lattice updates done by the GPU in the run, finished or not

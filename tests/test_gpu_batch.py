@@ -38,11 +38,12 @@ def lbmd3_for(spec, storage, rounding="nearest"):
     return lbm
 
 
+@pytest.mark.parametrize("q", [13, 15, 19, 27])
 @pytest.mark.parametrize("storage,rounding", [("f32", "nearest"), ("f16", "nearest"), ("f16", "stochastic")])
-def test_batch_case_equals_single_lattice(storage, rounding):
-    """With stochastic rounding the case's dither follows its own step count
-    from its seed, as the single-lattice solve's does."""
-    spec = sphere_case((26, 22, 30), steps=25)  # 17160 nodes: 4096-node state tiles
+def test_batch_case_equals_single_lattice(storage, rounding, q):
+    """Every lattice; with stochastic rounding the case's dither follows its
+    own step count from its seed, as the single-lattice solve's does."""
+    spec = sphere_case((26, 22, 30), q=q, steps=25)  # 17160 nodes: 4096-node state tiles
     spec.seed = 99
     lbm = lbmd3_for(spec, storage, rounding)
     spec.tau = lbm.flow.lat.tau  # the solver's tau comes from mu; use the same float32
