@@ -95,6 +95,31 @@ def test_alone_equals_mixed_bitwise(rounding):
             assert torch.equal(alone.particle_rows(a)[:, :15], mix.particle_rows(cid)[:, :15])
 
 
+def test_autotuned_grids_give_the_same_bits():
+    """The autotuner times real steps of the run with each candidate grid
+    (fused and marker kernels) and keeps the fastest; the cases then end
+    where a run with the default grids ends, bit for bit."""
+    specs = _mixed_specs()
+    for i, s in enumerate(specs):
+        s.seed = 2000 + i
+    tuned = BatchEngine(storage="f16", max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8)
+    plain = BatchEngine(storage="f16", max_nodes=1 << 17, max_markers=8192, max_particles=16, max_cases=8)
+    ids_t = [tuned.add(s) for s in specs]
+    ids_p = [plain.add(s) for s in specs]
+    res = tuned.autotune(steps=2)
+    used = res["steps_used"]
+    assert 0 < used < 40
+    assert str(res["grid_tiles"]) in res["ms_per_step"] and str(res["grid_markers"]) in res["ms_per_step_markers"]
+    tuned.step(40 - used)
+    plain.step(40)
+    for a, b, s in zip(ids_t, ids_p, specs):
+        ut, rt = tuned.fields(a)
+        up, rp = plain.fields(b)
+        assert torch.equal(ut, up) and torch.equal(rt, rp)
+        if s.particles:
+            assert torch.equal(tuned.particle_rows(a)[:, :15], plain.particle_rows(b)[:, :15])
+
+
 def test_cases_without_moment_boxes_equal_boxed_ones():
     """A cache too small for any box (every marker interpolated per node, from
     the miss list) gives the bits of the boxed interpolation; so does a cache
