@@ -39,7 +39,8 @@ def lbmd3_for(spec, storage, rounding="nearest"):
 
 
 @pytest.mark.parametrize("q", [13, 15, 19, 27])
-@pytest.mark.parametrize("storage,rounding", [("f32", "nearest"), ("f16", "nearest"), ("f16", "stochastic")])
+@pytest.mark.parametrize("storage,rounding", [("f32", "nearest"), ("f16", "nearest"), ("f16", "stochastic"),
+                                              ("bf16", "stochastic"), ("f64", "nearest")])
 def test_batch_case_equals_single_lattice(storage, rounding, q):
     """Every lattice; with stochastic rounding the case's dither follows its
     own step count from its seed, as the single-lattice solve's does."""
