@@ -56,11 +56,11 @@ def step_both(lbm, st, parts, size, steps, gl):
         ibo.run_step(st, parts, size, dt=1.0, gl=gl, step=s)
 
 
-@pytest.mark.parametrize("fused", [True, False])
-def test_sphere_matches_oracle(fused):
+@pytest.mark.parametrize("q,fused", [(19, True), (19, False), (13, True), (15, True), (27, True)])
+def test_sphere_matches_oracle(q, fused):
     size = (23, 23, 31)
-    lbm = make_case(particles=SPHERE, fused=fused)
-    st, parts = oracle_case(lbm)
+    lbm = make_case(q=q, particles=SPHERE, fused=fused)
+    st, parts = oracle_case(lbm, q=q)
     step_both(lbm, st, parts, list(size), 30, 2e-5)
     p, o = lbm.particle_swarm.particles[0], parts[0]
     assert np.allclose(p.cx.cpu().numpy(), o.cx, rtol=0, atol=1e-6)
