@@ -220,12 +220,22 @@ def test_boxed_interpolation_equals_per_node_pull(q, storage):
     start at the origin (step 1: every lane outside the boxes), then lie in
     them, with stencils clamped at a wall and boxes that overlap (two
     particles claim shared nodes)."""
+    _boxed_equals_per_node(q, storage, "cosine")
+
+
+@pytest.mark.parametrize("q,storage", [(19, "f32"), (27, "f16")])
+def test_boxed_interpolation_with_peskin_kernel(q, storage):
+    """The same with Peskin's 4-point kernel for the weights."""
+    _boxed_equals_per_node(q, storage, "peskin4")
+
+
+def _boxed_equals_per_node(q, storage, delta):
     size = (27, 23, 23)
     bcs = (W, W, P, P, P, P)
     runs = []
     for boxed in (True, False):
         lbm = make_case(q=q, size=size, mu=0.05 / 3, bcs=bcs, particles=BOX_PARTS, gl=1e-5,
-                        extra_flow={"storage": storage})
+                        extra_flow={"storage": storage, "ibm_delta": delta})
         lbm.boxed_interp = boxed
         if boxed:
             assert lbm._box_chunks is not None and lbm._box_nodes > 0
